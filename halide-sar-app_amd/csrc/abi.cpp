@@ -1,0 +1,519 @@
+// The reference's Halide-buffer entry points (backprojection, backprojection_cuda,
+// img_output_to_dB, img_output_u8, dft_init_fftw, dft_destroy_fftw) over device-resident plans.
+//
+// Each call validates the buffers the way Halide-generated code would (types, dimensions, regions),
+// answers bounds queries, packs the host inputs into pinned staging memory, runs the pipeline on the
+// GPU and scatters the result into the caller's (possibly strided, possibly partial) output.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "internal.h"
+
+using namespace sarbp;
+
+namespace {
+
+bool is_bq(const halide_buffer_t *b) { return b->host == nullptr && b->device == 0; }
+
+bool has_type(const halide_buffer_t *b, uint8_t code, uint8_t bits) {
+    return b->type.code == code && b->type.bits == bits && b->type.lanes == 1;
+}
+
+// Byte offset of coordinate c[] within a buffer (Halide: sum (c_i - min_i) * stride_i elements).
+inline int64_t offset_of(const halide_buffer_t *b, const int64_t *c) {
+    int64_t off = 0;
+    for (int i = 0; i < b->dimensions; ++i)
+        off += (c[i] - b->dim[i].min) * (int64_t)b->dim[i].stride;
+    return off;
+}
+
+inline bool in_dim(const halide_buffer_t *b, int i, int64_t c) {
+    return c >= b->dim[i].min && c < (int64_t)b->dim[i].min + b->dim[i].extent;
+}
+
+bool covers(const halide_buffer_t *b, int i, int64_t lo, int64_t hi) {
+    return b->dim[i].min <= lo && (int64_t)b->dim[i].min + b->dim[i].extent >= hi;
+}
+
+void set_dense(halide_buffer_t *b, std::initializer_list<std::pair<int, int>> mins_exts) {
+    int i = 0;
+    int64_t stride = 1;
+    for (auto me : mins_exts) {
+        b->dim[i].min = me.first;
+        b->dim[i].extent = me.second;
+        b->dim[i].stride = (int32_t)stride;
+        stride *= me.second;
+        ++i;
+    }
+}
+
+// Growable device / pinned-host buffers of one device.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    bool pinned = false;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            if (pinned) (void)hipHostFree(p);
+            else (void)hipFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = bytes + bytes / 8;
+        hipError_t e = pinned ? hipHostMalloc(&p, want, hipHostMallocDefault) : hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
+        else p = nullptr;
+        return e;
+    }
+};
+
+struct Context {
+    std::mutex mu;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf d_phs, d_kr, d_pos, d_r, d_out, d_aux;
+    DevBuf h_in{nullptr, 0, true}, h_out{nullptr, 0, true};
+    std::map<std::tuple<int, int, int, int, int, int, int, int, int, int, int, int, long long,
+                        long long, double>,
+             sarbp_plan *>
+        plans;
+    ~Context() {
+        for (auto &kv : plans) sarbp_plan_destroy(kv.second);
+    }
+};
+
+std::mutex g_ctx_mu;
+std::map<int, std::unique_ptr<Context>> g_ctx;
+
+int get_context(Context **out) {
+    int ndev = sarbp_device_count();
+    if (ndev <= 0) {
+        set_error("no HIP device is visible; the HIP backprojection path cannot run");
+        return halide_error_code_no_device_interface;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    auto &slot = g_ctx[dev];
+    if (!slot) {
+        auto c = std::make_unique<Context>();
+        c->device = dev;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            set_error("stream creation failed");
+            return halide_error_code_device_run_failed;
+        }
+        slot = std::move(c);
+    }
+    *out = slot.get();
+    return 0;
+}
+
+int hip_fail(hipError_t e, int code, const char *what) {
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return code;
+}
+
+int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t taylor_s_l,
+                        int32_t N_fft, double delta_r, halide_buffer_t *u, halide_buffer_t *v,
+                        halide_buffer_t *pos, halide_buffer_t *r, halide_buffer_t *out) {
+    halide_buffer_t *all[] = {phs, k_r, u, v, pos, r, out};
+    const char *names[] = {"phs", "k_r", "u", "v", "pos", "r", "output_img"};
+    const int dims[] = {3, 1, 1, 1, 2, 2, 3};
+    const uint8_t codes[] = {halide_type_float, halide_type_float, halide_type_float,
+                             halide_type_float, halide_type_float, halide_type_float,
+                             halide_type_float};
+    const uint8_t bits[] = {32, 32, 64, 64, 32, 64, 64};
+    for (int i = 0; i < 7; ++i) {
+        if (!all[i]) {
+            set_error("buffer argument %s is NULL", names[i]);
+            return halide_error_code_buffer_argument_is_null;
+        }
+        if (all[i]->dimensions != dims[i] || !all[i]->dim) {
+            set_error("%s has %d dimensions, expected %d", names[i], all[i]->dimensions, dims[i]);
+            return halide_error_code_bad_dimensions;
+        }
+        if (!has_type(all[i], codes[i], bits[i])) {
+            set_error("%s has type code %d bits %d lanes %d, expected float%d", names[i],
+                      all[i]->type.code, all[i]->type.bits, all[i]->type.lanes, bits[i]);
+            return halide_error_code_bad_type;
+        }
+        for (int dd = 0; dd < dims[i]; ++dd) {
+            if (all[i]->dim[dd].extent < 0) {
+                set_error("%s has a negative extent in dimension %d", names[i], dd);
+                return halide_error_code_buffer_extents_negative;
+            }
+        }
+    }
+    // backprojection.cpp:69-72: extents define the problem
+    const int ns = phs->dim[1].extent, np = phs->dim[2].extent;
+    const int nu = u->dim[0].extent, nv = v->dim[0].extent;
+    const int64_t npix = (int64_t)nu * nv;
+
+    // Bounds query (dft.cpp:49-58 protocol): fill the required regions and return.
+    bool bq = false;
+    if (is_bq(k_r)) { set_dense(k_r, {{0, ns}}); bq = true; }
+    if (is_bq(pos)) { set_dense(pos, {{0, 3}, {0, np}}); bq = true; }
+    if (is_bq(r)) {
+        if (npix > INT32_MAX) {
+            set_error("nu * nv overflows the pixel_locs extent");
+            return halide_error_code_buffer_extents_too_large;
+        }
+        set_dense(r, {{0, (int)npix}, {0, 3}});
+        bq = true;
+    }
+    if (is_bq(phs)) { set_dense(phs, {{0, 2}, {0, ns}, {0, np}}); bq = true; }
+    if (is_bq(u) || is_bq(v) || is_bq(out)) bq = true;
+    if (bq) return 0;
+
+    // Required regions (Halide's access checks)
+    if (!covers(phs, 0, 0, 2) || phs->dim[1].min != 0 || phs->dim[2].min != 0) {
+        set_error("phs must cover {[0,2), [0,nsamples), [0,npulses)}");
+        return halide_error_code_access_out_of_bounds;
+    }
+    if (!covers(k_r, 0, 0, ns) || !covers(k_r, 0, ns / 2, ns / 2 + 1)) {
+        set_error("k_r must cover [0, nsamples)");
+        return halide_error_code_access_out_of_bounds;
+    }
+    if (out->dim[0].min < 0 || out->dim[0].min + out->dim[0].extent > 2) {
+        set_error("output_img dimension 0 must lie within [0, 2) (real, imaginary)");
+        return halide_error_code_access_out_of_bounds;
+    }
+    if (N_fft > kMaxFft || N_fft < 2 || (N_fft & (N_fft - 1))) {
+        set_error("N_fft (%d) must be a power of two in [2, %d]", N_fft, kMaxFft);
+        return N_fft > kMaxFft ? halide_error_code_param_too_large
+                               : halide_error_code_param_too_small;
+    }
+    const int ox0 = out->dim[1].min, oy0 = out->dim[2].min;
+    const int ow = out->dim[1].extent, oh = out->dim[2].extent;
+    if (ow == 0 || oh == 0 || out->dim[0].extent == 0) return 0;
+
+    Context *ctx = nullptr;
+    int rv = get_context(&ctx);
+    if (rv) return rv;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+
+    int norm_policy, den_policy;
+    sarbp_get_default_policies(&norm_policy, &den_policy);
+    sarbp_plan_desc d;
+    sarbp_plan_desc_init(&d);
+    d.nsamples = ns;
+    d.npulses = np;
+    d.n_fft = N_fft;
+    d.taylor_s_l = taylor_s_l;
+    d.delta_r = delta_r;
+    d.nu = nu;
+    d.nv = nv;
+    d.x0 = ox0;
+    d.y0 = oy0;
+    d.width = ow;
+    d.height = oh;
+    d.r_min = r->dim[0].min;
+    d.r_extent = r->dim[0].extent;
+    d.norm_r0_policy = norm_policy;
+    d.taylor_den_policy = den_policy;
+    d.device = ctx->device;
+    auto key = std::make_tuple(ns, np, N_fft, taylor_s_l, nu, nv, ox0, oy0, ow, oh, norm_policy,
+                               den_policy, (long long)d.r_min, (long long)d.r_extent, delta_r);
+    sarbp_plan *plan = nullptr;
+    auto it = ctx->plans.find(key);
+    if (it != ctx->plans.end()) {
+        plan = it->second;
+    } else {
+        if (ctx->plans.size() >= 8) {
+            (void)hipStreamSynchronize(ctx->stream);
+            for (auto &kv : ctx->plans) sarbp_plan_destroy(kv.second);
+            ctx->plans.clear();
+        }
+        rv = sarbp_plan_create(&d, &plan);
+        if (rv) return rv;
+        ctx->plans[key] = plan;
+    }
+
+    // Pack inputs into pinned staging: phs | k_r | pos | r (each 256-byte aligned).
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t b_phs = align(sizeof(float) * 2 * (size_t)ns * np);
+    const size_t b_kr = align(sizeof(float) * (size_t)ns);
+    const size_t b_pos = align(sizeof(float) * 3 * (size_t)np);
+    const int64_t rext = r->dim[0].extent;
+    const size_t b_r = align(sizeof(double) * 3 * (size_t)rext);
+    hipError_t e = ctx->h_in.ensure(b_phs + b_kr + b_pos + b_r);
+    if (e != hipSuccess) return hip_fail(e, halide_error_code_out_of_memory, "pinned staging");
+    uint8_t *h = (uint8_t *)ctx->h_in.p;
+    float *hphs = (float *)h;
+    float *hkr = (float *)(h + b_phs);
+    float *hpos = (float *)(h + b_phs + b_kr);
+    double *hr = (double *)(h + b_phs + b_kr + b_pos);
+    {
+        const float *src = (const float *)phs->host;
+        const bool dense = phs->dim[0].stride == 1 && phs->dim[1].stride == 2 &&
+                           phs->dim[2].stride == 2 * ns && phs->dim[0].min == 0;
+        if (dense) {
+            std::memcpy(hphs, src, sizeof(float) * 2 * (size_t)ns * np);
+        } else {
+            for (int64_t p = 0; p < np; ++p)
+                for (int64_t s = 0; s < ns; ++s)
+                    for (int64_t c = 0; c < 2; ++c) {
+                        const int64_t co[3] = {c, s, p};
+                        hphs[(p * ns + s) * 2 + c] = src[offset_of(phs, co)];
+                    }
+        }
+        const float *ks = (const float *)k_r->host;
+        for (int64_t s = 0; s < ns; ++s) {
+            const int64_t co[1] = {s};
+            hkr[s] = ks[offset_of(k_r, co)];
+        }
+        // pos = constant_exterior(pos_in, 0.0f) (backprojection.cpp:76)
+        const float *ps = (const float *)pos->host;
+        for (int64_t p = 0; p < np; ++p)
+            for (int64_t c = 0; c < 3; ++c) {
+                const int64_t co[2] = {c, p};
+                hpos[p * 3 + c] =
+                    (in_dim(pos, 0, c) && in_dim(pos, 1, p)) ? ps[offset_of(pos, co)] : 0.0f;
+            }
+        // r = constant_exterior(r_in, 0.0) (backprojection.cpp:77); pixels outside r's region are
+        // handled by the kernel through (r_min, r_extent)
+        const double *rs = (const double *)r->host;
+        const int64_t rmin = r->dim[0].min;
+        for (int64_t c = 0; c < 3; ++c) {
+            double *dst = hr + c * rext;
+            if (!in_dim(r, 1, c)) {
+                std::memset(dst, 0, sizeof(double) * (size_t)rext);
+                continue;
+            }
+            if (r->dim[0].stride == 1) {
+                const int64_t co[2] = {rmin, c};
+                std::memcpy(dst, rs + offset_of(r, co), sizeof(double) * (size_t)rext);
+            } else {
+                for (int64_t i = 0; i < rext; ++i) {
+                    const int64_t co[2] = {rmin + i, c};
+                    dst[i] = rs[offset_of(r, co)];
+                }
+            }
+        }
+    }
+    const size_t b_out = sizeof(double) * 2 * (size_t)ow * oh;
+    if ((e = ctx->d_aux.ensure(b_phs + b_kr + b_pos + b_r)) != hipSuccess ||
+        (e = ctx->d_out.ensure(b_out)) != hipSuccess)
+        return hip_fail(e, halide_error_code_device_malloc_failed, "device staging");
+    uint8_t *dbase = (uint8_t *)ctx->d_aux.p;
+    if ((e = hipMemcpyAsync(dbase, h, b_phs + b_kr + b_pos + b_r, hipMemcpyHostToDevice,
+                            ctx->stream)) != hipSuccess)
+        return hip_fail(e, halide_error_code_copy_to_device_failed, "input upload");
+    rv = sarbp_plan_execute(plan, dbase, dbase + b_phs, dbase + b_phs + b_kr,
+                            dbase + b_phs + b_kr + b_pos, ctx->d_out.p, ctx->stream);
+    if (rv) return rv;
+    if ((e = ctx->h_out.ensure(b_out)) != hipSuccess)
+        return hip_fail(e, halide_error_code_out_of_memory, "pinned output staging");
+    if ((e = hipMemcpyAsync(ctx->h_out.p, ctx->d_out.p, b_out, hipMemcpyDeviceToHost,
+                            ctx->stream)) != hipSuccess)
+        return hip_fail(e, halide_error_code_copy_to_host_failed, "output download");
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return hip_fail(e, halide_error_code_device_run_failed, "pipeline execution");
+
+    // Scatter {2, ow, oh} into the caller's output region.
+    const double *res = (const double *)ctx->h_out.p;
+    double *dst = (double *)out->host;
+    const int c0 = out->dim[0].min, c1 = out->dim[0].min + out->dim[0].extent;
+    const bool dense = c0 == 0 && c1 == 2 && out->dim[0].stride == 1 && out->dim[1].stride == 2 &&
+                       out->dim[2].stride == 2 * ow;
+    if (dense) {
+        std::memcpy(dst, res, b_out);
+    } else {
+        for (int64_t y = 0; y < oh; ++y)
+            for (int64_t x = 0; x < ow; ++x)
+                for (int64_t c = c0; c < c1; ++c) {
+                    const int64_t co[3] = {c, ox0 + x, oy0 + y};
+                    dst[offset_of(out, co)] = res[(y * ow + x) * 2 + c];
+                }
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int backprojection(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t taylor_s_l,
+                              int32_t N_fft, double delta_r, halide_buffer_t *u, halide_buffer_t *v,
+                              halide_buffer_t *pos, halide_buffer_t *r,
+                              halide_buffer_t *output_img) {
+    return backprojection_impl(phs, k_r, taylor_s_l, N_fft, delta_r, u, v, pos, r, output_img);
+}
+
+extern "C" int backprojection_cuda(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t taylor_s_l,
+                                   int32_t N_fft, double delta_r, halide_buffer_t *u,
+                                   halide_buffer_t *v, halide_buffer_t *pos, halide_buffer_t *r,
+                                   halide_buffer_t *output_img) {
+    return backprojection_impl(phs, k_r, taylor_s_l, N_fft, delta_r, u, v, pos, r, output_img);
+}
+
+extern "C" int img_output_to_dB(halide_buffer_t *img, halide_buffer_t *out) {
+    if (!img || !out) {
+        set_error("img_output_to_dB: NULL buffer");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    if (img->dimensions != 3 || out->dimensions != 2 || !img->dim || !out->dim) {
+        set_error("img_output_to_dB: expected img {2, nu, nv} and out {nu, nv}");
+        return halide_error_code_bad_dimensions;
+    }
+    if (!has_type(img, halide_type_float, 64) || !has_type(out, halide_type_float, 64)) {
+        set_error("img_output_to_dB: img and out must be float64");
+        return halide_error_code_bad_type;
+    }
+    // img_output.cpp:23: RDom r(0, img.dim(1).extent(), 0, img.dim(2).extent())
+    const int W = img->dim[1].extent, H = img->dim[2].extent;
+    if (is_bq(img)) {
+        set_dense(img, {{0, 2}, {0, W}, {0, H}});
+        return 0;
+    }
+    if (is_bq(out)) return 0;
+    if (W < 0 || H < 0 || out->dim[0].extent < 0 || out->dim[1].extent < 0) {
+        set_error("img_output_to_dB: negative extent");
+        return halide_error_code_buffer_extents_negative;
+    }
+    if (!covers(img, 0, 0, 2) || img->dim[1].min != 0 || img->dim[2].min != 0 ||
+        !covers(img, 1, out->dim[0].min, (int64_t)out->dim[0].min + out->dim[0].extent) ||
+        !covers(img, 2, out->dim[1].min, (int64_t)out->dim[1].min + out->dim[1].extent)) {
+        set_error("img_output_to_dB: img must cover [0,2) x [0,W) x [0,H) and the output region");
+        return halide_error_code_access_out_of_bounds;
+    }
+    const int64_t ix0 = img->dim[1].min, iy0 = img->dim[2].min;
+    const int64_t IW = img->dim[1].extent, IH = img->dim[2].extent;
+    const int64_t npix = IW * IH;
+    if (npix == 0) return 0;
+    Context *ctx = nullptr;
+    int rv = get_context(&ctx);
+    if (rv) return rv;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipError_t e;
+    // Upload the whole img buffer region dense: the max runs over [0, W) x [0, H), which is the
+    // region itself when mins are 0 (required above unless the region is larger).
+    if ((e = ctx->h_in.ensure(sizeof(double) * 2 * npix)) != hipSuccess ||
+        (e = ctx->h_out.ensure(sizeof(double) * npix)) != hipSuccess)
+        return hip_fail(e, halide_error_code_out_of_memory, "pinned staging");
+    if ((e = ctx->d_aux.ensure(sizeof(double) * 2 * npix)) != hipSuccess ||
+        (e = ctx->d_out.ensure(sizeof(double) * npix)) != hipSuccess)
+        return hip_fail(e, halide_error_code_device_malloc_failed, "device staging");
+    double *h = (double *)ctx->h_in.p;
+    const double *src = (const double *)img->host;
+    // Only the RDom's [0, W) x [0, H) contributes to the max: pixels of a larger region outside it
+    // are packed as zeros, which cannot raise a max of magnitudes.
+    for (int64_t y = 0; y < IH; ++y)
+        for (int64_t x = 0; x < IW; ++x) {
+            const int64_t gx = ix0 + x, gy = iy0 + y;
+            const bool in_rdom = gx >= 0 && gx < W && gy >= 0 && gy < H;
+            for (int64_t c = 0; c < 2; ++c) {
+                const int64_t co[3] = {c, gx, gy};
+                h[(y * IW + x) * 2 + c] = in_rdom ? src[offset_of(img, co)] : 0.0;
+            }
+        }
+    if ((e = hipMemcpyAsync(ctx->d_aux.p, h, sizeof(double) * 2 * npix, hipMemcpyHostToDevice,
+                            ctx->stream)) != hipSuccess)
+        return hip_fail(e, halide_error_code_copy_to_device_failed, "img upload");
+    rv = sarbp_img_output_device(ctx->d_aux.p, npix, (double *)ctx->d_out.p, nullptr, 0.0, 1.0,
+                                 ctx->stream);
+    if (rv) return rv;
+    if ((e = hipMemcpyAsync(ctx->h_out.p, ctx->d_out.p, sizeof(double) * npix,
+                            hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess)
+        return hip_fail(e, halide_error_code_copy_to_host_failed, "dB download");
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return hip_fail(e, halide_error_code_device_run_failed, "dB execution");
+    // Values for pixels outside the RDom but inside img's region are recomputed with the true
+    // magnitude: dB depends on the pixel and the global max only.
+    const double *res = (const double *)ctx->h_out.p;
+    double *dst = (double *)out->host;
+    for (int64_t y = out->dim[1].min; y < (int64_t)out->dim[1].min + out->dim[1].extent; ++y)
+        for (int64_t x = out->dim[0].min; x < (int64_t)out->dim[0].min + out->dim[0].extent; ++x) {
+            const int64_t co[2] = {x, y};
+            dst[offset_of(out, co)] = res[(y - iy0) * IW + (x - ix0)];
+        }
+    return 0;
+}
+
+extern "C" int img_output_u8(halide_buffer_t *dB, double dB_min, double dB_max,
+                             halide_buffer_t *out) {
+    if (!dB || !out) {
+        set_error("img_output_u8: NULL buffer");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    if (dB->dimensions != 2 || out->dimensions != 2 || !dB->dim || !out->dim) {
+        set_error("img_output_u8: expected 2-D buffers");
+        return halide_error_code_bad_dimensions;
+    }
+    if (!has_type(dB, halide_type_float, 64) || !has_type(out, halide_type_uint, 8)) {
+        set_error("img_output_u8: dB must be float64 and out uint8");
+        return halide_error_code_bad_type;
+    }
+    if (is_bq(dB)) {
+        set_dense(dB, {{out->dim[0].min, out->dim[0].extent}, {out->dim[1].min, out->dim[1].extent}});
+        return 0;
+    }
+    if (is_bq(out)) return 0;
+    const int64_t x0 = out->dim[0].min, y0 = out->dim[1].min;
+    const int64_t W = out->dim[0].extent, H = out->dim[1].extent;
+    if (W < 0 || H < 0) {
+        set_error("img_output_u8: negative extent");
+        return halide_error_code_buffer_extents_negative;
+    }
+    if (!covers(dB, 0, x0, x0 + W) || !covers(dB, 1, y0, y0 + H)) {
+        set_error("img_output_u8: dB must cover the output region");
+        return halide_error_code_access_out_of_bounds;
+    }
+    const int64_t npix = W * H;
+    if (npix == 0) return 0;
+    Context *ctx = nullptr;
+    int rv = get_context(&ctx);
+    if (rv) return rv;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipError_t e;
+    if ((e = ctx->h_in.ensure(sizeof(double) * npix)) != hipSuccess ||
+        (e = ctx->h_out.ensure((size_t)npix)) != hipSuccess)
+        return hip_fail(e, halide_error_code_out_of_memory, "pinned staging");
+    if ((e = ctx->d_aux.ensure(sizeof(double) * npix)) != hipSuccess ||
+        (e = ctx->d_out.ensure((size_t)npix)) != hipSuccess)
+        return hip_fail(e, halide_error_code_device_malloc_failed, "device staging");
+    double *h = (double *)ctx->h_in.p;
+    const double *src = (const double *)dB->host;
+    for (int64_t y = 0; y < H; ++y)
+        for (int64_t x = 0; x < W; ++x) {
+            const int64_t co[2] = {x0 + x, y0 + y};
+            h[y * W + x] = src[offset_of(dB, co)];
+        }
+    if ((e = hipMemcpyAsync(ctx->d_aux.p, h, sizeof(double) * npix, hipMemcpyHostToDevice,
+                            ctx->stream)) != hipSuccess)
+        return hip_fail(e, halide_error_code_copy_to_device_failed, "dB upload");
+    if ((e = launch_u8_only((const double *)ctx->d_aux.p, npix, (uint8_t *)ctx->d_out.p, dB_min,
+                            dB_max, ctx->stream)) != hipSuccess)
+        return hip_fail(e, halide_error_code_device_run_failed, "u8 launch");
+    if ((e = hipMemcpyAsync(ctx->h_out.p, ctx->d_out.p, (size_t)npix, hipMemcpyDeviceToHost,
+                            ctx->stream)) != hipSuccess)
+        return hip_fail(e, halide_error_code_copy_to_host_failed, "u8 download");
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return hip_fail(e, halide_error_code_device_run_failed, "u8 execution");
+    const uint8_t *res = (const uint8_t *)ctx->h_out.p;
+    uint8_t *dst = out->host;
+    for (int64_t y = 0; y < H; ++y)
+        for (int64_t x = 0; x < W; ++x) {
+            const int64_t co[2] = {x0 + x, y0 + y};
+            dst[offset_of(out, co)] = res[y * W + x];
+        }
+    return 0;
+}
+
+// dft.cpp:20-30 builds the process-wide FFTW plan; here it builds the device twiddle table.
+extern "C" void dft_init_fftw(size_t N_fft) {
+    if (N_fft < 2 || N_fft > (size_t)kMaxFft || (N_fft & (N_fft - 1))) return;
+    if (sarbp_device_count() <= 0) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    (void)twiddles(dev, (int)N_fft, nullptr);
+}
+
+// dft.cpp:32-35 destroys the plan. Twiddle tables are shared by every plan of the process and are
+// kept until it exits, so that calls in flight on other threads are unaffected.
+extern "C" void dft_destroy_fftw(void) {}

@@ -1,0 +1,543 @@
+// HIP kernels of the backprojection hot path for gfx950 (MI355X), and their launchers.
+//
+//   pulse_const_kernel      ||pos_p|| table + per-pulse constants (backprojection.cpp:108,120-123)
+//   range_profile_kernel    window * |k_r|, centred pad, fftshift, LDS-resident radix-4/2 Stockham
+//                           FFT, fftshift -> range profiles Q (backprojection.cpp:80-104, dft.cpp)
+//   accumulate_kernel       per-pixel range, linear interpolation of Q, phase-corrected sum over
+//                           pulses, mid-pulse phase and row flip (backprojection.cpp:52-65,108-138)
+//   absmax / db_u8 kernels  img_output_to_dB and img_output_u8 (img_output.cpp:10-52)
+//   sim_phs_kernel          seeded point-scatterer phase histories for benchmarks and tests
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "internal.h"
+
+namespace sarbp {
+
+namespace {
+
+__device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ inline int floordiv2(int a) { return (a >= 0) ? a / 2 : -((-a + 1) / 2); }
+
+template <typename QT>
+__device__ inline float2 loadq(const QT *p);
+template <>
+__device__ inline float2 loadq<float2>(const float2 *p) {
+    return *p;
+}
+template <>
+__device__ inline float2 loadq<__half2>(const __half2 *p) {
+    return __half22float2(*p);
+}
+
+template <typename QT>
+__device__ inline void storeq(QT *p, float2 v);
+template <>
+__device__ inline void storeq<float2>(float2 *p, float2 v) {
+    *p = v;
+}
+template <>
+__device__ inline void storeq<__half2>(__half2 *p, float2 v) {
+    *p = __float22half2_rn(v);
+}
+
+// sqrt(s) from the hardware reciprocal square root (about 23 correct bits) and one
+// Newton-Raphson step in f64: relative error ~1.5 * (2^-23)^2, below 1e-13.
+__device__ inline double fast_sqrt(double s) {
+    double y = __builtin_amdgcn_rsq(s);
+    double g = s * y;
+    double h = 0.5 * y;
+    double r = __builtin_fma(-g, h, 0.5);
+    return __builtin_fma(g, r, g);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ per-pulse constants
+
+__global__ void pulse_const_kernel(const float *__restrict__ pos, const float *__restrict__ k_r,
+                                   int ns, int np, double lsa, double inv_lsr, int policy,
+                                   PulseConst *__restrict__ pc, BpHeader *__restrict__ hdr) {
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    // k_c = k_r(nsamples / 2), f32 (backprojection.cpp:123)
+    const double k_c = (double)k_r[ns / 2];
+    const double gamma = k_c / (2.0 * M_PI);
+    if (p == 0) {
+        hdr->k_c = k_c;
+        hdr->gamma = gamma;
+        hdr->lsa = lsa;
+        hdr->inv_lsr = inv_lsr;
+    }
+    if (p >= np) return;
+    const float x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
+    double nr0;
+    if (policy == SARBP_NORM_R0_F64) {
+        const double dx = x, dy = y, dz = z;
+        nr0 = __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)),
+                                   __dmul_rn(dz, dz)));
+    } else if (policy == SARBP_NORM_R0_F32_FMA) {
+        nr0 = (double)__fsqrt_rn(__fmaf_rn(z, z, __fmaf_rn(y, y, __fmul_rn(x, x))));
+    } else {
+        // norm3d of the f32 pos Func (util.h:64-68): f32 products and sums, IEEE, in order
+        const float s = __fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z));
+        nr0 = (double)__fsqrt_rn(s);
+    }
+    PulseConst c;
+    c.px = x;
+    c.py = y;
+    c.pz = z;
+    c.C = c.px * c.px + c.py * c.py + c.pz * c.pz;
+    c.B = (nr0 - lsa) * inv_lsr;
+    c.G = -gamma * nr0;
+    c.nr0 = nr0;
+    c.pad = 0.0;
+    pc[p] = c;
+}
+
+// ------------------------------------------------------------------ range profiles + FFT
+
+constexpr int kFftThreads = 512;
+
+template <int R>
+__device__ inline void dft_r(float2 *v);
+
+template <>
+__device__ inline void dft_r<2>(float2 *v) {
+    const float2 a = v[0], b = v[1];
+    v[0] = make_float2(a.x + b.x, a.y + b.y);
+    v[1] = make_float2(a.x - b.x, a.y - b.y);
+}
+
+// forward 4-point DFT, exp(-2 pi i r k / 4)
+template <>
+__device__ inline void dft_r<4>(float2 *v) {
+    const float2 a0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+    const float2 a1 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const float2 a2 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+    const float2 d = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+    const float2 a3 = make_float2(d.y, -d.x);  // -i * (v1 - v3)
+    v[0] = make_float2(a0.x + a2.x, a0.y + a2.y);
+    v[2] = make_float2(a0.x - a2.x, a0.y - a2.y);
+    v[1] = make_float2(a1.x + a3.x, a1.y + a3.y);
+    v[3] = make_float2(a1.x - a3.x, a1.y - a3.y);
+}
+
+// One Stockham autosort pass of radix R over the LDS-resident row, in place: every thread loads
+// its butterflies' inputs, the block synchronises, then results are written to their sorted slots.
+template <int R, int NB>
+__device__ inline void fft_pass(float2 *buf, int N, int Ns, const float2 *__restrict__ tw,
+                                int tid) {
+    const int nbf = N / R;
+    const int stride = N / (Ns * R);
+    float2 v[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = tid + b * kFftThreads;
+        if (j < nbf) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * nbf];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = tid + b * kFftThreads;
+        if (j < nbf) {
+            const int k = j & (Ns - 1);
+            if (k != 0) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * k * stride]);
+            }
+            dft_r<R>(v[b]);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// One block per range profile p. Source pulse s = (p + 2 ceil(np/2)) mod np: the pulse axis is
+// rolled by fftshift before and after the FFT (backprojection.cpp:98,104).
+template <typename QT>
+__global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
+    const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
+    const double *__restrict__ wp, int ns, int np, int N, int logN,
+    const float2 *__restrict__ tw, QT *__restrict__ q) {
+    extern __shared__ float2 buf[];
+    const int p = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int pmid = np / 2 + np % 2;
+    const int s = (int)(((long long)p + 2LL * pmid) % np);
+    const int half = N / 2;                 // ceil(N/2) for even N
+    const int xpad = floordiv2(N - ns);     // pad's leading offset (signal_complex.h:14)
+    const double wps = wp[s];
+    const float2 *row = phs + (size_t)s * ns;
+    for (int x = tid; x < N; x += kFftThreads) {
+        const int xs = (x + half) & (N - 1);  // fftshift (signal_complex.h:24-29)
+        const int i = xs - xpad;
+        float2 v = make_float2(0.f, 0.f);
+        if (i >= 0 && i < ns) {
+            const float2 a = row[i];
+            const float f = fabsf(k_r[i]);      // filt = |k_r| (backprojection.cpp:85)
+            const double w = ws[i] * wps;       // win (backprojection.cpp:82)
+            // phs * filt rounds in f32, then the f64 window applies (backprojection.cpp:90)
+            v.x = (float)((double)(a.x * f) * w);
+            v.y = (float)((double)(a.y * f) * w);
+        }
+        buf[x] = v;
+    }
+    __syncthreads();
+    int Ns = 1;
+    if (logN & 1) {
+        fft_pass<2, kMaxFft / 2 / kFftThreads>(buf, N, 1, tw, tid);
+        Ns = 2;
+    }
+    for (; Ns < N; Ns *= 4) fft_pass<4, kMaxFft / 4 / kFftThreads>(buf, N, Ns, tw, tid);
+    QT *dst = q + (size_t)p * N;
+    for (int k = tid; k < N; k += kFftThreads) storeq<QT>(dst + k, buf[(k + half) & (N - 1)]);
+}
+
+// ------------------------------------------------------------------ accumulation
+
+constexpr int kBpThreads = 256;
+constexpr int kBpPulseBlock = 64;
+
+// Tile of 16 x (16 * PPT) output pixels per block; thread (tx, ty) owns pixels (tx, ty + 16 j).
+// Each pixel keeps -2 r and |r|^2 in f64 registers; per pulse:
+//   d     = sqrt(|r|^2 + |pos|^2 - 2 r.pos)                  f64
+//   luts  = B_p - d / lsr ; i = floor(luts) ; w = luts - i   f64 (index), f32 (weight)
+//   q     = lerp(Q[clamp(i)], Q[clamp(i + 1)], w)            f32 complex (gathered from L1/L2)
+//   phase = fract(gamma * d + G_p)                           f64 -> f32 revolutions
+//   acc  += q * (cos, sin)(2 pi phase)                       f32, flushed to f64 every 64 pulses
+template <typename QT, int PPT>
+__global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int bx = blockIdx.x * 16, by = blockIdx.y * (16 * PPT);
+    const double inv_lsr = a.hdr->inv_lsr;
+    const double gamma = a.hdr->gamma;
+    const int N = a.n_fft;
+    const int np = a.np;
+
+    double m2x[PPT], m2y[PPT], m2z[PPT], r2[PPT];
+    bool ok[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const int lx = bx + tx, ly = by + ty + 16 * j;
+        ok[j] = lx < a.width && ly < a.height;
+        const int x = a.x0 + lx, y = a.y0 + ly;
+        // xy_to_pixel = nu * (nv - y - 1) + x (backprojection.cpp:130); r is constant_exterior 0
+        const int64_t px = (int64_t)a.nu * (int64_t)(a.nv - y - 1) + x;
+        const int64_t idx = px - a.r_min;
+        double rx = 0.0, ry = 0.0, rz = 0.0;
+        if (ok[j] && idx >= 0 && idx < a.r_extent) {
+            rx = a.r[idx];
+            ry = a.r[a.r_extent + idx];
+            rz = a.r[2 * a.r_extent + idx];
+        }
+        m2x[j] = -2.0 * rx;
+        m2y[j] = -2.0 * ry;
+        m2z[j] = -2.0 * rz;
+        r2[j] = rx * rx + ry * ry + rz * rz;
+    }
+
+    double accr[PPT], acci[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) accr[j] = acci[j] = 0.0;
+
+    const QT *qbase = reinterpret_cast<const QT *>(a.q);
+    for (int p0 = 0; p0 < np; p0 += kBpPulseBlock) {
+        float sr[PPT], si[PPT];
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) sr[j] = si[j] = 0.f;
+        const int pe = min(np, p0 + kBpPulseBlock);
+        for (int p = p0; p < pe; ++p) {
+            const PulseConst c = a.pc[p];
+            const QT *row = qbase + (size_t)p * N;
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) {
+                const double s2 = __builtin_fma(
+                    m2x[j], c.px, __builtin_fma(m2y[j], c.py, __builtin_fma(m2z[j], c.pz, r2[j] + c.C)));
+                const double d = fast_sqrt(s2);
+                const double luts = __builtin_fma(-inv_lsr, d, c.B);
+                const double fl = __builtin_floor(luts);
+                const int i = (int)fl;
+                const float wt = (float)(luts - fl);
+                const int i0 = min(max(i, 0), N - 1);
+                const int i1 = min(max(i + 1, 0), N - 1);
+                const float2 q0 = loadq<QT>(row + i0);
+                const float2 q1 = loadq<QT>(row + i1);
+                const float qr = __builtin_fmaf(wt, q1.x - q0.x, q0.x);
+                const float qi = __builtin_fmaf(wt, q1.y - q0.y, q0.y);
+                const double ph = __builtin_fma(gamma, d, c.G);
+                const float f = (float)__builtin_amdgcn_fract(ph);
+                const float sn = __builtin_amdgcn_sinf(f);
+                const float cs = __builtin_amdgcn_cosf(f);
+                sr[j] = __builtin_fmaf(qr, cs, sr[j]);
+                sr[j] = __builtin_fmaf(-qi, sn, sr[j]);
+                si[j] = __builtin_fmaf(qr, sn, si[j]);
+                si[j] = __builtin_fmaf(qi, cs, si[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            accr[j] += (double)sr[j];
+            acci[j] += (double)si[j];
+        }
+    }
+
+    // fimg = img * expj(k_c * dr_i(pixel, npulses / 2)) (backprojection.cpp:135)
+    const PulseConst cm = a.pc[np / 2];
+    const double k_c = a.hdr->k_c;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        if (!ok[j]) continue;
+        const double rx = -0.5 * m2x[j], ry = -0.5 * m2y[j], rz = -0.5 * m2z[j];
+        const double dx = rx - cm.px, dy = ry - cm.py, dz = rz - cm.pz;
+        const double dm = sqrt(dx * dx + dy * dy + dz * dz);
+        const double ang = k_c * (cm.nr0 - dm);
+        double sn, cs;
+        sincos(ang, &sn, &cs);
+        const int lx = bx + tx, ly = by + ty + 16 * j;
+        a.out[(size_t)ly * a.width + lx] =
+            make_double2(accr[j] * cs - acci[j] * sn, accr[j] * sn + acci[j] * cs);
+    }
+}
+
+// ------------------------------------------------------------------ image output
+
+constexpr int kImgThreads = 256;
+
+__global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
+
+// max |z| over the image (img_output.cpp:24); |z| >= 0, so the f64 bit pattern orders like the value
+__global__ __launch_bounds__(kImgThreads) void absmax_kernel(const double2 *__restrict__ img,
+                                                             int64_t n,
+                                                             unsigned long long *__restrict__ out) {
+    __shared__ double red[kImgThreads / 64];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 z = img[i];
+        const double a = sqrt(z.x * z.x + z.y * z.y);
+        m = a > m ? a : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kImgThreads / 64; ++w) m = red[w] > m ? red[w] : m;
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__device__ inline uint8_t to_u8(double dB, double lo, double hi) {
+    // normalize (util.h:13-15), dB_scale (signal.h:55-57), u8 truncation (img_output.cpp:50)
+    double c = dB < hi ? dB : hi;
+    c = c > lo ? c : lo;
+    double s = (c - lo) / (hi - lo) * 255.0;
+    s = s < 255.0 ? s : 255.0;
+    s = s > 0.0 ? s : 0.0;
+    return (s == s) ? (uint8_t)s : (uint8_t)0;
+}
+
+__global__ __launch_bounds__(kImgThreads) void db_u8_kernel(
+    const double2 *__restrict__ img, int64_t n, const unsigned long long *__restrict__ maxbits,
+    double ln10f, double *__restrict__ dB, uint8_t *__restrict__ u8, double lo, double hi) {
+    const double m = __longlong_as_double((long long)*maxbits);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 z = img[i];
+        const double a = sqrt(z.x * z.x + z.y * z.y);
+        // 10 * log10(|z| / m), log10(x) = log(x) / log(Expr(10)) with the f32 log(10) (util.h:56-58)
+        const double v = 10.0 * (log(a / m) / ln10f);
+        if (dB) dB[i] = v;
+        if (u8) u8[i] = to_u8(v, lo, hi);
+    }
+}
+
+__global__ __launch_bounds__(kImgThreads) void u8_kernel(const double *__restrict__ dB, int64_t n,
+                                                         uint8_t *__restrict__ u8, double lo,
+                                                         double hi) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        u8[i] = to_u8(dB[i], lo, hi);
+}
+
+// ------------------------------------------------------------------ scene simulator
+
+constexpr int kSimThreads = 256;
+
+__global__ __launch_bounds__(kSimThreads) void sim_phs_kernel(
+    const double *__restrict__ pos, int np, const double *__restrict__ k64, int ns,
+    const double *__restrict__ tgt, const double *__restrict__ amp, int nt,
+    float2 *__restrict__ phs) {
+    __shared__ double D[kSimThreads];
+    __shared__ float2 A[kSimThreads];
+    const int p = blockIdx.y;
+    const int k = blockIdx.x * kSimThreads + threadIdx.x;
+    const double px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
+    const double r0 = sqrt(px * px + py * py + pz * pz);
+    const double kr = k < ns ? k64[k] * (1.0 / (2.0 * M_PI)) : 0.0;
+    float accr = 0.f, acci = 0.f;
+    for (int t0 = 0; t0 < nt; t0 += kSimThreads) {
+        const int t = t0 + threadIdx.x;
+        if (t < nt) {
+            const double dx = px - tgt[3 * t], dy = py - tgt[3 * t + 1], dz = pz - tgt[3 * t + 2];
+            D[threadIdx.x] = r0 - sqrt(dx * dx + dy * dy + dz * dz);
+            A[threadIdx.x] = make_float2((float)amp[2 * t], (float)amp[2 * t + 1]);
+        }
+        __syncthreads();
+        const int tn = min(kSimThreads, nt - t0);
+        for (int u = 0; u < tn; ++u) {
+            const float f = (float)__builtin_amdgcn_fract(kr * D[u]);
+            const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
+            const float2 am = A[u];
+            accr += am.x * cs - am.y * sn;
+            acci += am.x * sn + am.y * cs;
+        }
+        __syncthreads();
+    }
+    if (k < ns) phs[(size_t)p * ns + k] = make_float2(accr, acci);
+}
+
+// ------------------------------------------------------------------ launchers
+
+hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np, int n_fft,
+                               double delta_r, int norm_policy, PulseConst *pc, BpHeader *hdr,
+                               hipStream_t stream) {
+    const RangeAxis ax = range_axis(ns, n_fft, delta_r);
+    const int threads = 256;
+    const int blocks = (np + threads - 1) / threads;
+    hipLaunchKernelGGL(pulse_const_kernel, dim3(blocks > 0 ? blocks : 1), dim3(threads), 0, stream,
+                       pos, k_r, ns, np, ax.lsa, ax.inv_lsr, norm_policy, pc, hdr);
+    return hipGetLastError();
+}
+
+hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
+                                 const double *wp, int ns, int np, int n_fft, const float2 *tw,
+                                 void *q, int q_format, hipStream_t stream) {
+    int logN = 0;
+    while ((1 << logN) < n_fft) ++logN;
+    const size_t lds = sizeof(float2) * (size_t)n_fft;
+    if (q_format == SARBP_Q_C16) {
+        static std::once_flag once;
+        std::call_once(once, [] {
+            (void)hipFuncSetAttribute((const void *)&range_profile_kernel<__half2>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+        });
+        hipLaunchKernelGGL(range_profile_kernel<__half2>, dim3(np), dim3(kFftThreads), lds, stream,
+                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (__half2 *)q);
+    } else {
+        static std::once_flag once;
+        std::call_once(once, [] {
+            (void)hipFuncSetAttribute((const void *)&range_profile_kernel<float2>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+        });
+        hipLaunchKernelGGL(range_profile_kernel<float2>, dim3(np), dim3(kFftThreads), lds, stream,
+                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (float2 *)q);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream) {
+    constexpr int PPT = 4;
+    const dim3 grid((a.width + 15) / 16, (a.height + 16 * PPT - 1) / (16 * PPT));
+    if (a.width <= 0 || a.height <= 0) return hipSuccess;
+    if (a.q_format == SARBP_Q_C16)
+        hipLaunchKernelGGL((accumulate_kernel<__half2, PPT>), grid, dim3(kBpThreads), 0, stream, a);
+    else
+        hipLaunchKernelGGL((accumulate_kernel<float2, PPT>), grid, dim3(kBpThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_img_output(const double2 *img, int64_t npix, double *dB, uint8_t *u8,
+                             double lo, double hi, unsigned long long *scratch_max,
+                             hipStream_t stream) {
+    if (npix <= 0) return hipSuccess;
+    const int64_t want = (npix + kImgThreads - 1) / kImgThreads;
+    const int blocks = (int)(want < 4096 ? want : 4096);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, stream, scratch_max);
+    hipLaunchKernelGGL(absmax_kernel, dim3(blocks), dim3(kImgThreads), 0, stream, img, npix,
+                       scratch_max);
+    const double ln10f = (double)std::log(10.0f);
+    hipLaunchKernelGGL(db_u8_kernel, dim3(blocks), dim3(kImgThreads), 0, stream, img, npix,
+                       scratch_max, ln10f, dB, u8, lo, hi);
+    return hipGetLastError();
+}
+
+hipError_t launch_u8_only(const double *dB, int64_t npix, uint8_t *u8, double lo, double hi,
+                          hipStream_t stream) {
+    if (npix <= 0) return hipSuccess;
+    const int64_t want = (npix + kImgThreads - 1) / kImgThreads;
+    const int blocks = (int)(want < 4096 ? want : 4096);
+    hipLaunchKernelGGL(u8_kernel, dim3(blocks), dim3(kImgThreads), 0, stream, dB, npix, u8, lo,
+                       hi);
+    return hipGetLastError();
+}
+
+hipError_t launch_sim_phs(const double *pos, int np, const double *k64, int ns,
+                          const double *tgt, const double *amp, int nt, float2 *phs,
+                          hipStream_t stream) {
+    if (np <= 0 || ns <= 0) return hipSuccess;
+    const dim3 grid((ns + kSimThreads - 1) / kSimThreads, np);
+    hipLaunchKernelGGL(sim_phs_kernel, grid, dim3(kSimThreads), 0, stream, pos, np, k64, ns, tgt,
+                       amp, nt, phs);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ twiddle cache
+
+namespace {
+std::mutex g_tw_mu;
+std::map<std::pair<int, int>, float2 *> g_tw;
+}  // namespace
+
+const float2 *twiddles(int device, int n_fft, hipStream_t stream) {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto key = std::make_pair(device, n_fft);
+    auto it = g_tw.find(key);
+    if (it != g_tw.end()) return it->second;
+    std::vector<float2> h((size_t)n_fft);
+    for (int t = 0; t < n_fft; ++t) {
+        const double a = -2.0 * M_PI * (double)t / (double)n_fft;
+        h[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    float2 *d = nullptr;
+    if (hipMalloc(&d, sizeof(float2) * (size_t)n_fft) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), sizeof(float2) * (size_t)n_fft, hipMemcpyHostToDevice) !=
+        hipSuccess) {
+        (void)hipFree(d);
+        return nullptr;
+    }
+    (void)stream;
+    g_tw[key] = d;
+    return d;
+}
+
+void release_twiddles() {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto &kv : g_tw) {
+        (void)hipSetDevice(kv.first.first);
+        (void)hipFree(kv.second);
+    }
+    (void)hipSetDevice(cur);
+    g_tw.clear();
+}
+
+}  // namespace sarbp

@@ -1,0 +1,298 @@
+// Device-resident plans: the windows, range-profile buffer and pulse table of one problem shape,
+// reused across calls. The Halide-buffer entry points and the serving engine both run on plans.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "internal.h"
+
+using namespace sarbp;
+
+struct sarbp_plan {
+    sarbp_plan_desc d;
+    double *ws = nullptr;       // Taylor window over samples, f64 [ns]
+    double *wp = nullptr;       // Taylor window over pulses, f64 [np]
+    void *q = nullptr;          // range profiles [np][N] in q_format
+    PulseConst *pc = nullptr;   // per-pulse constants [np]
+    BpHeader *hdr = nullptr;    // launch-wide constants
+    const float2 *tw = nullptr; // twiddles exp(-2 pi i t / N)
+};
+
+namespace {
+
+std::mutex g_policy_mu;
+int g_norm_policy = SARBP_NORM_R0_F32_STRICT;
+int g_den_policy = SARBP_TAYLOR_DEN_INT;
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+int check_desc(const sarbp_plan_desc *d) {
+    if (!d) {
+        set_error("plan descriptor is NULL");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    if (d->nsamples < 2 || d->npulses < 2) {
+        set_error("nsamples (%d) and npulses (%d) must be >= 2: the Taylor window's linspace "
+                  "divides by num - 1 (util.h:80-84)",
+                  d->nsamples, d->npulses);
+        return halide_error_code_param_too_small;
+    }
+    if (!is_pow2(d->n_fft) || d->n_fft < 2 || d->n_fft > kMaxFft) {
+        set_error("N_fft (%d) must be a power of two in [2, %d]", d->n_fft, kMaxFft);
+        return d->n_fft > kMaxFft ? halide_error_code_param_too_large
+                                  : halide_error_code_param_too_small;
+    }
+    if (d->nu < 1 || d->nv < 1 || d->width < 0 || d->height < 0 || d->r_extent < 0) {
+        set_error("image extents must be positive and region extents non-negative");
+        return halide_error_code_buffer_extents_negative;
+    }
+    if ((int64_t)d->nu * d->nv > INT32_MAX) {
+        set_error("nu * nv exceeds the 32-bit extent of the pixel_locs buffer");
+        return halide_error_code_buffer_extents_too_large;
+    }
+    if (d->norm_r0_policy < 0 || d->norm_r0_policy > 2 || d->taylor_den_policy < 0 ||
+        d->taylor_den_policy > 1 || d->q_format < 0 || d->q_format > 1) {
+        set_error("invalid policy or format in plan descriptor");
+        return halide_error_code_constraint_violated;
+    }
+    if (d->range_mode != SARBP_RANGE_F64) {
+        set_error("range_mode %d is not available; use SARBP_RANGE_F64", d->range_mode);
+        return halide_error_code_constraint_violated;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int sarbp_set_default_policies(int norm_r0_policy, int taylor_den_policy) {
+    if (norm_r0_policy < 0 || norm_r0_policy > 2 || taylor_den_policy < 0 || taylor_den_policy > 1) {
+        set_error("invalid policy (%d, %d)", norm_r0_policy, taylor_den_policy);
+        return halide_error_code_constraint_violated;
+    }
+    std::lock_guard<std::mutex> lk(g_policy_mu);
+    g_norm_policy = norm_r0_policy;
+    g_den_policy = taylor_den_policy;
+    return 0;
+}
+
+extern "C" int sarbp_get_default_policies(int *norm_r0_policy, int *taylor_den_policy) {
+    std::lock_guard<std::mutex> lk(g_policy_mu);
+    if (norm_r0_policy) *norm_r0_policy = g_norm_policy;
+    if (taylor_den_policy) *taylor_den_policy = g_den_policy;
+    return 0;
+}
+
+extern "C" int sarbp_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" void sarbp_plan_desc_init(sarbp_plan_desc *d) {
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    int np_, dp_;
+    sarbp_get_default_policies(&np_, &dp_);
+    d->norm_r0_policy = np_;
+    d->taylor_den_policy = dp_;
+    d->q_format = SARBP_Q_C32;
+    d->range_mode = SARBP_RANGE_F64;
+}
+
+extern "C" int sarbp_plan_destroy(sarbp_plan *p) {
+    if (!p) return 0;
+    DeviceGuard g(p->d.device);
+    (void)hipFree(p->ws);
+    (void)hipFree(p->wp);
+    (void)hipFree(p->q);
+    (void)hipFree(p->pc);
+    (void)hipFree(p->hdr);
+    delete p;
+    return 0;
+}
+
+extern "C" int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out) {
+    if (!out) {
+        set_error("sarbp_plan_create: out is NULL");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    *out = nullptr;
+    int rv = check_desc(d);
+    if (rv) return rv;
+    int ndev = sarbp_device_count();
+    if (ndev <= 0) {
+        set_error("no HIP device is visible");
+        return halide_error_code_no_device_interface;
+    }
+    if (d->device < 0 || d->device >= ndev) {
+        set_error("device %d out of range (%d visible)", d->device, ndev);
+        return halide_error_code_no_device_interface;
+    }
+    DeviceGuard g(d->device);
+    sarbp_plan *p = new sarbp_plan();
+    p->d = *d;
+    const int ns = d->nsamples, np = d->npulses, N = d->n_fft;
+    std::vector<double> ws((size_t)ns), wp((size_t)np);
+    taylor_window(ns, d->taylor_s_l, d->taylor_den_policy, ws.data());
+    taylor_window(np, d->taylor_s_l, d->taylor_den_policy, wp.data());
+    const size_t qelem = d->q_format == SARBP_Q_C16 ? 4 : 8;
+    bool ok = hipMalloc(&p->ws, sizeof(double) * ns) == hipSuccess &&
+              hipMalloc(&p->wp, sizeof(double) * np) == hipSuccess &&
+              hipMalloc(&p->q, qelem * (size_t)np * (size_t)N) == hipSuccess &&
+              hipMalloc(&p->pc, sizeof(PulseConst) * (size_t)np) == hipSuccess &&
+              hipMalloc(&p->hdr, sizeof(BpHeader)) == hipSuccess;
+    if (!ok) {
+        sarbp_plan_destroy(p);
+        set_error("device allocation failed for plan (np=%d, N=%d)", np, N);
+        return halide_error_code_device_malloc_failed;
+    }
+    if (hipMemcpy(p->ws, ws.data(), sizeof(double) * ns, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->wp, wp.data(), sizeof(double) * np, hipMemcpyHostToDevice) != hipSuccess) {
+        sarbp_plan_destroy(p);
+        set_error("copy of Taylor windows to device failed");
+        return halide_error_code_copy_to_device_failed;
+    }
+    p->tw = twiddles(d->device, N, nullptr);
+    if (!p->tw) {
+        sarbp_plan_destroy(p);
+        set_error("twiddle table allocation failed");
+        return halide_error_code_device_malloc_failed;
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" void *sarbp_plan_q_buffer(sarbp_plan *p) { return p ? p->q : nullptr; }
+
+extern "C" int sarbp_plan_range_profiles(sarbp_plan *p, const void *phs_dev, const void *k_r_dev,
+                                         void *q_dev, void *stream) {
+    if (!p || !phs_dev || !k_r_dev) {
+        set_error("sarbp_plan_range_profiles: NULL argument");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    DeviceGuard g(p->d.device);
+    hipError_t e = launch_range_profiles(phs_dev, (const float *)k_r_dev, p->ws, p->wp,
+                                         p->d.nsamples, p->d.npulses, p->d.n_fft, p->tw,
+                                         q_dev ? q_dev : p->q, p->d.q_format, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_error("range-profile launch failed: %s", hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    return 0;
+}
+
+extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const void *k_r_dev,
+                                     const void *pos_dev, const void *r_dev, void *out_dev,
+                                     void *stream) {
+    if (!p || !k_r_dev || !pos_dev || !out_dev || (!r_dev && p->d.r_extent > 0)) {
+        set_error("sarbp_plan_accumulate: NULL argument");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    DeviceGuard g(p->d.device);
+    const sarbp_plan_desc &d = p->d;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = launch_pulse_consts((const float *)pos_dev, (const float *)k_r_dev, d.nsamples,
+                                       d.npulses, d.n_fft, d.delta_r, d.norm_r0_policy, p->pc,
+                                       p->hdr, s);
+    if (e == hipSuccess) {
+        BpArgs a;
+        a.q = q_dev ? q_dev : p->q;
+        a.q_format = d.q_format;
+        a.n_fft = d.n_fft;
+        a.np = d.npulses;
+        a.pc = p->pc;
+        a.hdr = p->hdr;
+        a.r = (const double *)r_dev;
+        a.r_min = d.r_min;
+        a.r_extent = d.r_extent;
+        a.nu = d.nu;
+        a.nv = d.nv;
+        a.x0 = d.x0;
+        a.y0 = d.y0;
+        a.width = d.width;
+        a.height = d.height;
+        a.out = (double2 *)out_dev;
+        e = launch_accumulate(a, s);
+    }
+    if (e != hipSuccess) {
+        set_error("accumulation launch failed: %s", hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    return 0;
+}
+
+extern "C" int sarbp_plan_execute(sarbp_plan *p, const void *phs_dev, const void *k_r_dev,
+                                  const void *pos_dev, const void *r_dev, void *out_dev,
+                                  void *stream) {
+    int rv = sarbp_plan_range_profiles(p, phs_dev, k_r_dev, nullptr, stream);
+    if (rv) return rv;
+    return sarbp_plan_accumulate(p, nullptr, k_r_dev, pos_dev, r_dev, out_dev, stream);
+}
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<unsigned long long *> g_scratch;  // per-device 8-byte max cell
+}  // namespace
+
+extern "C" int sarbp_img_output_device(const void *img_dev, int64_t npix, double *dB_dev,
+                                       uint8_t *u8_dev, double dB_min, double dB_max,
+                                       void *stream) {
+    if (!img_dev && npix > 0) {
+        set_error("sarbp_img_output_device: img is NULL");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("no HIP device is visible");
+        return halide_error_code_no_device_interface;
+    }
+    unsigned long long *cell = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1, nullptr);
+        if (!g_scratch[dev] && hipMalloc(&g_scratch[dev], sizeof(unsigned long long)) != hipSuccess) {
+            g_scratch[dev] = nullptr;
+            set_error("scratch allocation failed");
+            return halide_error_code_device_malloc_failed;
+        }
+        cell = g_scratch[dev];
+    }
+    hipError_t e = launch_img_output((const double2 *)img_dev, npix, dB_dev, u8_dev, dB_min, dB_max,
+                                     cell, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_error("image-output launch failed: %s", hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    return 0;
+}
+
+extern "C" int sarbp_sim_phs_device(const double *pos_dev, int32_t npulses,
+                                    const double *k_r64_dev, int32_t nsamples,
+                                    const double *tgt_dev, const double *amp_dev,
+                                    int32_t ntargets, void *phs_dev, void *stream) {
+    if (!pos_dev || !k_r64_dev || !phs_dev || (ntargets > 0 && (!tgt_dev || !amp_dev))) {
+        set_error("sarbp_sim_phs_device: NULL argument");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    hipError_t e = launch_sim_phs(pos_dev, npulses, k_r64_dev, nsamples, tgt_dev, amp_dev,
+                                  ntargets, (float2 *)phs_dev, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_error("simulator launch failed: %s", hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    return 0;
+}

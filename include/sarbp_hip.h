@@ -1,0 +1,156 @@
+/*
+ * sarbp_hip — MI355X-native SAR backprojection behind the C ABI of the reference's
+ * Halide-generated `backprojection` pipeline (ISI-apex/halide-sar-app).
+ *
+ * Part 1 declares the drop-in entry points, exactly the symbols and argument meaning a caller of the
+ * reference binds (sarbp.cpp:183-229,277,299,316; dft.h). Part 2 declares extensions that have no
+ * counterpart in the reference: device-resident plans for callers whose data already lives in HBM,
+ * and helpers used by the serving engine, the scene simulator and the host driver.
+ *
+ * All entry points return 0 on success and a negative halide_error_code_t value on failure; none
+ * aborts. Buffers are owned by the caller; inputs are read-only.
+ */
+#ifndef SARBP_HIP_H
+#define SARBP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "halide_buffer_compat.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ============================== Part 1: the reference's ABI ============================== */
+
+/*
+ * Replaces the AOT symbol generated from BackprojectionGenerator (backprojection.cpp:35-46,
+ * registered at :354), called at sarbp.cpp:277 (bounds query at :269).
+ *   phs        f32 {2, nsamples, npulses}          (PlatformData.cpp:238)
+ *   k_r        f32 {nsamples}                      (PlatformData.cpp:137)
+ *   taylor_s_l Taylor sidelobe level in dB         (backprojection.cpp:37)
+ *   N_fft      FFT length (power of two, 2..16384) (backprojection.cpp:38, sarbp.cpp:258)
+ *   delta_r    range resolution                    (backprojection.cpp:39)
+ *   u, v       f64 {nu}, {nv}; only extents used   (backprojection.cpp:71-72)
+ *   pos        f32 {3, npulses}                    (PlatformData.cpp:219)
+ *   r          f64 {nu*nv, 3} pixel locations      (ImgPlane.cpp:103)
+ *   output_img f64 {2, x-range, y-range}; any region of the nu x nv image (backprojection.cpp:46,138)
+ * A buffer with host == NULL and device == 0 is a bounds query: required input regions are written
+ * into its dims and 0 is returned (the protocol of dft.cpp:49-58).
+ */
+int backprojection(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t taylor_s_l, int32_t N_fft,
+                   double delta_r, halide_buffer_t *u, halide_buffer_t *v, halide_buffer_t *pos,
+                   halide_buffer_t *r, halide_buffer_t *output_img);
+
+/* Replaces `backprojection_cuda` (backprojection.cpp:356, selected at sarbp.cpp:198-200): the same
+ * pipeline; on this library both names run on the GPU. */
+int backprojection_cuda(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t taylor_s_l,
+                        int32_t N_fft, double delta_r, halide_buffer_t *u, halide_buffer_t *v,
+                        halide_buffer_t *pos, halide_buffer_t *r, halide_buffer_t *output_img);
+
+/* Replaces img_output_to_dB (img_output.cpp:10-35, registered :54), called at sarbp.cpp:299:
+ * out(x, y) = 10 * log10(|img(x, y)| / max|img|); img f64 {2, nu, nv}, out f64 {nu, nv}. */
+int img_output_to_dB(halide_buffer_t *img, halide_buffer_t *out);
+
+/* Replaces img_output_u8 (img_output.cpp:41-52, registered :55), called at sarbp.cpp:316:
+ * out = u8(clamp(255 * (clamp(dB, lo, hi) - lo) / (hi - lo), 0, 255)); dB f64 {nu, nv}, out u8. */
+int img_output_u8(halide_buffer_t *dB, double dB_min, double dB_max, halide_buffer_t *out);
+
+/* Replace dft_init_fftw / dft_destroy_fftw (dft.h, dft.cpp:20-35), called at sarbp.cpp:261,292.
+ * Here they build / release the device twiddle table for N_fft; calling them is optional. */
+void dft_init_fftw(size_t N_fft);
+void dft_destroy_fftw(void);
+
+/* ============================== Part 2: extensions ============================== */
+
+/* Numerical policies for the two places where the reference's result depends on how Halide
+ * compiled it (no counterpart in the reference; process-wide defaults for Part 1 calls). */
+enum sarbp_norm_r0_policy {
+    SARBP_NORM_R0_F32_STRICT = 0, /* ||pos|| in f32, IEEE, no contraction (default) */
+    SARBP_NORM_R0_F32_FMA = 1,    /* f32 with fma(z, z, fma(y, y, x*x)) */
+    SARBP_NORM_R0_F64 = 2,        /* f64 */
+};
+enum sarbp_taylor_den_policy {
+    SARBP_TAYLOR_DEN_INT = 0, /* (m+1)^2/(n+1)^2 as Int(32) division (default) */
+    SARBP_TAYLOR_DEN_F32 = 1, /* the same ratio in f32 */
+};
+enum sarbp_q_format { SARBP_Q_C32 = 0, SARBP_Q_C16 = 1 };
+enum sarbp_range_mode {
+    SARBP_RANGE_F64 = 0, /* per-pixel f64 |r - pos| (default) */
+    SARBP_RANGE_DF32 = 1 /* double-float (f32 pair) differential range */
+};
+
+int sarbp_set_default_policies(int norm_r0_policy, int taylor_den_policy);
+int sarbp_get_default_policies(int *norm_r0_policy, int *taylor_den_policy);
+
+/* Last error message of the calling thread ("" if none). */
+const char *sarbp_last_error(void);
+
+/* Number of visible GPUs (0 when none; never aborts). */
+int sarbp_device_count(void);
+
+/* Host-side Taylor window (signal.h:14-51) with an explicit policy; out has num doubles. */
+int sarbp_taylor_window(int32_t num, int32_t taylor_s_l, int32_t taylor_den_policy, double *out);
+
+/* Host-side image plane (ImgPlane.cpp:16-63, img_plane.cpp:12-113, ip.h:5-17).
+ * dims_out = {nu, nv}; scal_out = {d_u, d_v}. With u == NULL only dims/scal are written.
+ * pixel_locs (may be NULL) is f64 {nu*nv, 3} with stride (1, nu*nv). */
+int sarbp_img_plane_create(int32_t nsamples, int32_t npulses, double delta_r, const float *R_c,
+                           const float *n_hat, double res_factor, double aspect, int32_t upsample,
+                           int32_t *dims_out, double *scal_out, double *u, double *v, double *k_u,
+                           double *k_v, double *u_hat, double *v_hat, double *pixel_locs);
+
+/* Device-resident plan: everything the hot path needs besides the per-call inputs. */
+typedef struct sarbp_plan sarbp_plan;
+
+typedef struct sarbp_plan_desc {
+    int32_t nsamples, npulses, n_fft, taylor_s_l;
+    double delta_r;
+    int32_t nu, nv;                  /* image-plane extents (only extents matter, like u/v) */
+    int32_t x0, y0, width, height;   /* output region of the nu x nv image */
+    int64_t r_min, r_extent;         /* pixel index range held by the r buffer */
+    int32_t norm_r0_policy, taylor_den_policy;
+    int32_t q_format;                /* sarbp_q_format */
+    int32_t range_mode;              /* sarbp_range_mode */
+    int32_t device;                  /* HIP device ordinal */
+    int32_t reserved[7];
+} sarbp_plan_desc;
+
+void sarbp_plan_desc_init(sarbp_plan_desc *d);
+int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out);
+int sarbp_plan_destroy(sarbp_plan *p);
+
+/* Run the whole pipeline on `stream` (hipStream_t, may be NULL for the null stream).
+ *   phs_dev   complex f32 [npulses][nsamples]       k_r_dev  f32 [nsamples]
+ *   pos_dev   f32 [npulses][3]                      r_dev    f64 [3][r_extent] (SoA)
+ *   out_dev   f64 {2, width, height} dense (interleaved complex, row y-major)
+ * Asynchronous: returns after enqueueing. */
+int sarbp_plan_execute(sarbp_plan *p, const void *phs_dev, const void *k_r_dev,
+                       const void *pos_dev, const void *r_dev, void *out_dev, void *stream);
+
+/* The two stages separately (for profiling and tests). q_dev may be NULL to use the plan's own
+ * range-profile buffer; otherwise it must hold npulses * n_fft complex values in q_format. */
+int sarbp_plan_range_profiles(sarbp_plan *p, const void *phs_dev, const void *k_r_dev,
+                              void *q_dev, void *stream);
+int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const void *k_r_dev,
+                          const void *pos_dev, const void *r_dev, void *out_dev, void *stream);
+/* Device pointer of the plan's range-profile buffer. */
+void *sarbp_plan_q_buffer(sarbp_plan *p);
+
+/* dB and u8 image output on the device for an f64 complex image of npix pixels. dB_dev and/or
+ * u8_dev may be NULL. Synchronous with respect to `stream` only. */
+int sarbp_img_output_device(const void *img_dev, int64_t npix, double *dB_dev, uint8_t *u8_dev,
+                            double dB_min, double dB_max, void *stream);
+
+/* Synthetic phase history on the device: phs[p][k] = sum_s amp_s exp(j k_r64[k] * D[p][s]) with
+ * D[p][s] = |pos_p| - |pos_p - tgt_s| in f64. pos f64 [np][3], tgt f64 [nt][3], amp complex f64
+ * [nt], k_r64 f64 [ns]; out complex f32 [np][ns]. */
+int sarbp_sim_phs_device(const double *pos_dev, int32_t npulses, const double *k_r64_dev,
+                         int32_t nsamples, const double *tgt_dev, const double *amp_dev,
+                         int32_t ntargets, void *phs_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

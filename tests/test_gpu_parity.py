@@ -1,0 +1,247 @@
+"""Parity of the HIP path (through the C ABI) with the CPU FP64 oracle.
+
+Tolerance: relative L2 error <= 1e-4 over complex pixels (||img - oracle|| / ||oracle||), the
+bar the north-star metric states; the measured error is ~1e-6 and is printed. Integer outputs (u8)
+are compared exactly given identical inputs.
+"""
+import importlib
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+TOL = 1e-4
+NTHREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a visible GPU"
+    return torch
+
+
+@pytest.fixture(scope="module")
+def pipe():
+    return importlib.import_module("halide-sar-app_amd.pipeline")
+
+
+def run_abi(sar, phs, k_r, taylor, n_fft, delta_r, nu, nv, pos, r, fn=None):
+    out = np.zeros((nv, nu), np.complex128)
+    u, v = np.zeros(nu), np.zeros(nv)
+    rv = (fn or sar.backprojection)(phs, k_r, taylor, n_fft, delta_r, u, v, pos, r, out)
+    assert rv == 0, sar.last_error()
+    return out
+
+
+@pytest.mark.parametrize("np_", [15, 16])
+def test_tiny_golden_all_policies(sar, torch_dev, np_):
+    g = np.load(os.path.join(GOLDEN, f"tiny_np{np_}.npz"))
+    args = (g["phs"], g["k_r"], int(g["taylor"]), int(g["n_fft"]), float(g["delta_r"]),
+            int(g["nu"]), int(g["nv"]), g["pos"], g["pixel_locs"])
+    try:
+        for pol, key in ((sar.NORM_R0_F32_STRICT, "img_strict"), (sar.NORM_R0_F32_FMA, "img_fma"),
+                         (sar.NORM_R0_F64, "img_f64")):
+            assert sar.set_default_policies(pol, sar.TAYLOR_DEN_INT) == 0
+            img = run_abi(sar, *args)
+            e = rel_l2(img, g[key])
+            print(f"np={np_} policy={key} rel_l2={e:.3e}")
+            assert e < TOL
+        for den, key in ((sar.TAYLOR_DEN_INT, "img_s20_int"), (sar.TAYLOR_DEN_F32, "img_s20_f32")):
+            assert sar.set_default_policies(sar.NORM_R0_F32_STRICT, den) == 0
+            a = list(args)
+            a[2] = 20
+            img = run_abi(sar, *a)
+            e = rel_l2(img, g[key])
+            print(f"np={np_} taylor20 {key} rel_l2={e:.3e}")
+            assert e < TOL
+    finally:
+        sar.set_default_policies(sar.NORM_R0_F32_STRICT, sar.TAYLOR_DEN_INT)
+
+
+def test_cuda_alias_and_determinism(sar, torch_dev):
+    g = np.load(os.path.join(GOLDEN, "tiny_np15.npz"))
+    args = (g["phs"], g["k_r"], int(g["taylor"]), int(g["n_fft"]), float(g["delta_r"]),
+            int(g["nu"]), int(g["nv"]), g["pos"], g["pixel_locs"])
+    a = run_abi(sar, *args)
+    b = run_abi(sar, *args, fn=sar.backprojection_cuda)
+    c = run_abi(sar, *args)
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_c1_full_image_vs_oracle(sar, orc, torch_dev):
+    g = np.load(os.path.join(GOLDEN, "c1_64.npz"))
+    ip = sar.img_plane_create(424, 469, float(g["delta_r"]), g["R_c"])
+    nu, nv = ip["nu"], ip["nv"]
+    img = run_abi(sar, g["phs"], g["k_r"], int(g["taylor"]), int(g["n_fft"]), float(g["delta_r"]),
+                  nu, nv, g["pos"], ip["pixel_locs"])
+    x0, y0, w, h = (int(t) for t in g["region"])
+    e_gold = rel_l2(img[y0:y0 + h, x0:x0 + w], g["img"])
+    ref = orc.backprojection(g["phs"], g["k_r"], int(g["taylor"]), int(g["n_fft"]),
+                             float(g["delta_r"]), nu, nv, g["pos"], ip["pixel_locs"],
+                             nthreads=NTHREADS)
+    e = rel_l2(img, ref)
+    print(f"C1 512x512x469: rel_l2 vs oracle {e:.3e}, vs golden centre {e_gold:.3e}")
+    assert e < TOL and e_gold < TOL
+
+
+def test_partial_region_and_strided_output(sar, torch_dev):
+    g = np.load(os.path.join(GOLDEN, "tiny_np16.npz"))
+    nu, nv = int(g["nu"]), int(g["nv"])
+    args = (g["phs"], g["k_r"], int(g["taylor"]), int(g["n_fft"]), float(g["delta_r"]))
+    full = run_abi(sar, *args, nu, nv, g["pos"], g["pixel_locs"])
+    u, v = np.zeros(nu), np.zeros(nv)
+    # a 7 x 5 window at (x, y) = (3, 4)
+    sub = np.zeros((5, 7), np.complex128)
+    ob = sar.Buffer(sub, halide_dims=[(0, 2, 1), (3, 7, 2), (4, 5, 14)])
+    assert sar.backprojection(*args, u, v, g["pos"], g["pixel_locs"], ob) == 0, sar.last_error()
+    assert np.array_equal(sub, full[4:9, 3:10])
+    # strided (transposed) output and only the real part
+    big = np.zeros((nu, nv, 2))
+    ob = sar.Buffer(big, halide_dims=[(0, 1, 1), (0, nu, 2 * nv), (0, nv, 2)])
+    assert sar.backprojection(*args, u, v, g["pos"], g["pixel_locs"], ob) == 0
+    assert np.array_equal(big[:, :, 0].T, full.real) and not big[:, :, 1].any()
+    # strided inputs: phs as a view of a wider array, pixel_locs with a non-unit stride
+    wide = np.zeros((g["phs"].shape[0], g["phs"].shape[1] + 5), np.complex64)
+    wide[:, 2:2 + g["phs"].shape[1]] = g["phs"]
+    phs_view = wide[:, 2:2 + g["phs"].shape[1]]
+    r2 = np.zeros((3, nu * nv, 2))
+    r2[:, :, 0] = g["pixel_locs"]
+    rb = sar.Buffer(r2, halide_dims=[(0, nu * nv, 2), (0, 3, 2 * nu * nv)], type_code=2, bits=64)
+    out = np.zeros((nv, nu), np.complex128)
+    rv = sar.backprojection(phs_view, g["k_r"], int(g["taylor"]), int(g["n_fft"]),
+                            float(g["delta_r"]), u, v, g["pos"], rb, out)
+    assert rv == 0 and np.array_equal(out, full)
+
+
+def test_pixel_locs_shorter_than_image_reads_zero(sar, orc, torch_dev):
+    """r is constant_exterior(r_in, 0) (backprojection.cpp:77): pixels past its extent sit at the
+    origin."""
+    g = np.load(os.path.join(GOLDEN, "tiny_np16.npz"))
+    nu, nv = int(g["nu"]), int(g["nv"])
+    short = np.ascontiguousarray(g["pixel_locs"][:, : nu * nv // 2])
+    img = run_abi(sar, g["phs"], g["k_r"], int(g["taylor"]), int(g["n_fft"]), float(g["delta_r"]),
+                  nu, nv, g["pos"], short)
+    padded = np.zeros_like(g["pixel_locs"])
+    padded[:, : nu * nv // 2] = short
+    ref = orc.backprojection(g["phs"], g["k_r"], int(g["taylor"]), int(g["n_fft"]),
+                             float(g["delta_r"]), nu, nv, g["pos"], padded)
+    assert rel_l2(img, ref) < TOL
+
+
+@pytest.mark.parametrize("ns,np_,n_fft", [(24, 2, 64), (24, 3, 64), (40, 9, 32), (33, 7, 128),
+                                          (2, 4, 2)])
+def test_edge_shapes(sar, orc, scenes, torch_dev, ns, np_, n_fft):
+    """Smallest pulse counts, odd sizes and N_fft < nsamples (the pad then crops the centre)."""
+    sc = scenes.tiny_scene(npulses=np_, nsamples=ns, seed=ns + np_)
+    ip = orc.img_plane(ns, np_, sc.delta_r, sc.R_c)
+    nu, nv = ip["nu"], ip["nv"]
+    img = run_abi(sar, sc.phs, sc.k_r, 17, n_fft, sc.delta_r, nu, nv, sc.pos, ip["pixel_locs"])
+    ref = orc.backprojection(sc.phs, sc.k_r, 17, n_fft, sc.delta_r, nu, nv, sc.pos,
+                             ip["pixel_locs"])
+    e = rel_l2(img, ref)
+    print(f"ns={ns} np={np_} N={n_fft}: rel_l2 {e:.3e}")
+    assert e < TOL
+
+
+def test_range_profiles_vs_oracle(sar, orc, scenes, torch_dev):
+    torch = torch_dev
+    sc = scenes.tiny_scene(npulses=37, nsamples=300, seed=3)
+    for n_fft in (512, 1024, 2048):
+        plan = sar.Plan(sc.nsamples, sc.npulses, n_fft, sc.taylor, sc.delta_r, 8, 8)
+        phs = torch.from_numpy(sc.phs).cuda()
+        k_r = torch.from_numpy(sc.k_r).cuda()
+        q = torch.empty((sc.npulses, n_fft), dtype=torch.complex64, device="cuda")
+        plan.range_profiles(phs, k_r, q)
+        torch.cuda.synchronize()
+        ref = orc.range_profiles(sc.phs, sc.k_r, sc.taylor, n_fft)
+        err = np.abs(q.cpu().numpy() - ref).max() / np.abs(ref).max()
+        print(f"Q N={n_fft}: max err / max |Q| = {err:.3e}")
+        assert err < 1e-5
+
+
+def test_c2_full_size_sampled_and_scaling(sar, orc, scenes, pipe, torch_dev):
+    """Configuration C2 at full size (2048^2 x 2000 pulses, N 8192): oracle on 9216 sampled pixels
+    (two full rows, two full columns, 1024 random ones) plus exact 2x linearity of the whole image."""
+    torch = torch_dev
+    sc = scenes.config_scene("C2")
+    grid = pipe.scene_grid(sc)
+    ds = pipe.DeviceScene(sc, grid)
+    plan = ds.plan()
+    out = ds.output()
+    ds.run(plan, out)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy()
+    assert np.all(np.isfinite(img))
+    phs = ds.phs.cpu().numpy()
+    nu, nv = grid["nu"], grid["nv"]
+    rng = np.random.default_rng(7)
+    ys = np.concatenate([np.zeros(nu, int), np.full(nu, nv // 2 + 3), np.arange(nv),
+                         np.arange(nv), rng.integers(0, nv, 1024)])
+    xs = np.concatenate([np.arange(nu), np.arange(nu), np.zeros(nv, int), np.full(nv, nu - 1),
+                         rng.integers(0, nu, 1024)])
+    Q = orc.range_profiles(phs, sc.k_r, sc.taylor, ds.n_fft)
+    _, nr0 = orc.norm_r0(sc.pos)
+    ref = orc.backproject_points(Q, sc.nsamples, sc.delta_r, sc.k_r[sc.nsamples // 2], sc.pos,
+                                 nr0, grid["pixel_locs"], nu, nv, xs, ys, nthreads=NTHREADS)
+    e = rel_l2(img[ys, xs], ref)
+    print(f"C2 2048^2 x 2000: sampled rel_l2 {e:.3e} over {len(xs)} pixels")
+    assert e < TOL
+    # scaling phs by 2 is exact in every f32/f64 operation of the path
+    phs2 = ds.phs * 2
+    out2 = ds.output()
+    plan.execute(phs2, ds.k_r, ds.pos, ds.r, out2)
+    torch.cuda.synchronize()
+    assert torch.equal(out2, out * 2)
+
+
+def test_img_output_parity(sar, orc, torch_dev):
+    torch = torch_dev
+    g = np.load(os.path.join(GOLDEN, "c1_64.npz"))
+    img = np.ascontiguousarray(g["img"])
+    dB = np.zeros(img.shape)
+    assert sar.img_output_to_dB(img, dB) == 0, sar.last_error()
+    ref = orc.img_output_to_dB(img)
+    assert np.abs(dB - ref).max() < 1e-9
+    u8 = np.zeros(img.shape, np.uint8)
+    assert sar.img_output_u8(ref, -30.0, 0.0, u8) == 0
+    assert np.array_equal(u8, orc.img_output_u8(ref, -30.0, 0.0))
+    # device-resident fused dB + u8
+    t = torch.from_numpy(img).cuda()
+    dd = torch.empty(img.shape, dtype=torch.float64, device="cuda")
+    du = torch.empty(img.shape, dtype=torch.uint8, device="cuda")
+    sar.img_output_device(t, img.size, dd, du, -30.0, 0.0)
+    torch.cuda.synchronize()
+    assert np.abs(dd.cpu().numpy() - ref).max() < 1e-9
+    mism = np.count_nonzero(du.cpu().numpy() != orc.img_output_u8(ref, -30.0, 0.0))
+    assert mism <= img.size * 1e-3
+    # partial output window of img_output_to_dB
+    sub = np.zeros((10, 12))
+    sb = sar.Buffer(sub, halide_dims=[(5, 12, 1), (20, 10, 12)])
+    assert sar.img_output_to_dB(img, sb) == 0
+    assert np.abs(sub - ref[20:30, 5:17]).max() < 1e-9
+
+
+def test_q_f16_storage_error(sar, orc, pipe, torch_dev, scenes):
+    torch = torch_dev
+    g = np.load(os.path.join(GOLDEN, "c1_64.npz"))
+    ip = sar.img_plane_create(424, 469, float(g["delta_r"]), g["R_c"])
+    sc = scenes.config_scene("C1")
+    sc.phs = g["phs"]
+    ds = pipe.DeviceScene(sc, ip)
+    region = tuple(int(t) for t in g["region"])
+    outs = []
+    for fmt in (sar.Q_C32, sar.Q_C16):
+        plan = ds.plan(region=region, q_format=fmt)
+        out = ds.output(region)
+        ds.run(plan, out)
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+    e32, e16 = rel_l2(outs[0], g["img"]), rel_l2(outs[1], g["img"])
+    print(f"C1 centre: Q c32 rel_l2 {e32:.3e}, Q c16 rel_l2 {e16:.3e}")
+    assert e32 < TOL and e16 < 5e-3

@@ -21,6 +21,14 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsarbp_hip.so")
 
+# One HIP runtime per process: torch ships its own libamdhip64 (soname libamdhip64.so.7). When it is
+# loaded first, the library's dependency on that soname binds to it; loaded the other way round the
+# process would hold two runtimes and the second cannot see the GPU.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         f"{LIB_PATH} is missing: build it with `make -C {_HERE}` (or __graft_entry__.build()); "
@@ -154,6 +162,7 @@ _lib.sarbp_plan_range_profiles.argtypes = [_P, _P, _P, _P, _P]
 _lib.sarbp_plan_accumulate.argtypes = [_P, _P, _P, _P, _P, _P, _P]
 _lib.sarbp_plan_q_buffer.argtypes = [_P]
 _lib.sarbp_plan_q_buffer.restype = _P
+_lib.sarbp_plan_read_pulse_norms.argtypes = [_P, _P, _i32]
 _lib.sarbp_img_output_device.argtypes = [_P, _i64, _P, _P, _f64, _f64, _P]
 _lib.sarbp_sim_phs_device.argtypes = [_P, _i32, _P, _i32, _P, _P, _i32, _P, _P]
 
@@ -356,6 +365,15 @@ class Plan:
                                         _stream_handle(stream))
         if rv != 0:
             raise RuntimeError(f"sarbp_plan_accumulate failed ({rv}): {last_error()}")
+
+    def pulse_norms(self, n=None) -> np.ndarray:
+        """The ||pos_p|| table of the last accumulate (synchronous)."""
+        n = self.desc.npulses if n is None else n
+        out = np.empty(n, np.float64)
+        rv = _lib.sarbp_plan_read_pulse_norms(self._p, out.ctypes.data_as(_P), n)
+        if rv != 0:
+            raise RuntimeError(f"sarbp_plan_read_pulse_norms failed ({rv}): {last_error()}")
+        return out
 
     @property
     def q_ptr(self) -> int:

@@ -49,10 +49,11 @@ hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int n
                                hipStream_t stream);
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
                                  const double *wp, int ns, int np, int n_fft, const float2 *tw,
-                                 void *q, int q_format, hipStream_t stream);
+                                 void *q, int q_format, float *qscale, hipStream_t stream);
 struct BpArgs {
     const void *q;
     int q_format;
+    const float *qscale;  // per-profile inverse scale of f16 profiles (NULL for f32)
     int n_fft, np;
     const PulseConst *pc;
     const BpHeader *hdr;

@@ -80,17 +80,18 @@ __global__ void pulse_const_kernel(const float *__restrict__ pos, const float *_
     }
     if (p >= np) return;
     const float x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
+    // Plain operators under `fp contract(off)` round each product and sum once; sqrtf and sqrt are
+    // the IEEE correctly rounded forms (the __fsqrt_rn intrinsic maps to the 1-ulp native sqrt).
     double nr0;
     if (policy == SARBP_NORM_R0_F64) {
         const double dx = x, dy = y, dz = z;
-        nr0 = __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)),
-                                   __dmul_rn(dz, dz)));
+        nr0 = sqrt((dx * dx + dy * dy) + dz * dz);
     } else if (policy == SARBP_NORM_R0_F32_FMA) {
-        nr0 = (double)__fsqrt_rn(__fmaf_rn(z, z, __fmaf_rn(y, y, __fmul_rn(x, x))));
+        nr0 = (double)sqrtf(fmaf(z, z, fmaf(y, y, x * x)));
     } else {
-        // norm3d of the f32 pos Func (util.h:64-68): f32 products and sums, IEEE, in order
-        const float s = __fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z));
-        nr0 = (double)__fsqrt_rn(s);
+        // norm3d of the f32 pos Func (util.h:64-68): f32 products and sums, in order
+        const float xx = x * x, yy = y * y, zz = z * z;
+        nr0 = (double)sqrtf((xx + yy) + zz);
     }
     PulseConst c;
     c.px = x;
@@ -173,8 +174,9 @@ template <typename QT>
 __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
     const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
     const double *__restrict__ wp, int ns, int np, int N, int logN,
-    const float2 *__restrict__ tw, QT *__restrict__ q) {
+    const float2 *__restrict__ tw, QT *__restrict__ q, float *__restrict__ qscale) {
     extern __shared__ float2 buf[];
+    __shared__ float red[kFftThreads / 64];
     const int p = blockIdx.x;
     const int tid = threadIdx.x;
     const int pmid = np / 2 + np % 2;
@@ -205,7 +207,27 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
     }
     for (; Ns < N; Ns *= 4) fft_pass<4, kMaxFft / 4 / kFftThreads>(buf, N, Ns, tw, tid);
     QT *dst = q + (size_t)p * N;
-    for (int k = tid; k < N; k += kFftThreads) storeq<QT>(dst + k, buf[(k + half) & (N - 1)]);
+    float scale = 1.0f;
+    if (qscale) {
+        // f16 storage: scale the profile by 2^e so that its largest component is about 2^14
+        // (exact to undo; 11-bit significands keep ~5e-4 relative rounding per value)
+        float m = 0.0f;
+        for (int k = tid; k < N; k += kFftThreads) m = fmaxf(m, fmaxf(fabsf(buf[k].x), fabsf(buf[k].y)));
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        if ((tid & 63) == 0) red[tid >> 6] = m;
+        __syncthreads();
+        m = red[0];
+        for (int w = 1; w < kFftThreads / 64; ++w) m = fmaxf(m, red[w]);
+        int e = 0;
+        if (m > 0.0f && m <= 3.0e38f) (void)frexpf(m, &e);
+        e = max(-100, min(100, 14 - e));
+        scale = ldexpf(1.0f, e);
+        if (tid == 0) qscale[p] = ldexpf(1.0f, -e);
+    }
+    for (int k = tid; k < N; k += kFftThreads) {
+        float2 v = buf[(k + half) & (N - 1)];
+        storeq<QT>(dst + k, make_float2(v.x * scale, v.y * scale));
+    }
 }
 
 // ------------------------------------------------------------------ accumulation
@@ -264,6 +286,7 @@ __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
         for (int p = p0; p < pe; ++p) {
             const PulseConst c = a.pc[p];
             const QT *row = qbase + (size_t)p * N;
+            const float qs = a.qscale ? a.qscale[p] : 1.0f;
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
                 const double s2 = __builtin_fma(
@@ -277,8 +300,12 @@ __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
                 const int i1 = min(max(i + 1, 0), N - 1);
                 const float2 q0 = loadq<QT>(row + i0);
                 const float2 q1 = loadq<QT>(row + i1);
-                const float qr = __builtin_fmaf(wt, q1.x - q0.x, q0.x);
-                const float qi = __builtin_fmaf(wt, q1.y - q0.y, q0.y);
+                float qr = __builtin_fmaf(wt, q1.x - q0.x, q0.x);
+                float qi = __builtin_fmaf(wt, q1.y - q0.y, q0.y);
+                if (a.qscale) {
+                    qr *= qs;
+                    qi *= qs;
+                }
                 const double ph = __builtin_fma(gamma, d, c.G);
                 const float f = (float)__builtin_amdgcn_fract(ph);
                 const float sn = __builtin_amdgcn_sinf(f);
@@ -427,28 +454,35 @@ hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int n
     return hipGetLastError();
 }
 
+// Allows a kernel the largest dynamic LDS the CU offers next to its static LDS (160 KiB total on
+// gfx950). Failures are cleared so that they do not surface from a later hipGetLastError.
+static void allow_max_dynamic_lds(const void *fn) {
+    hipFuncAttributes attr;
+    size_t stat = 0;
+    if (hipFuncGetAttributes(&attr, fn) == hipSuccess) stat = attr.sharedSizeBytes;
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(163840 - stat));
+    (void)hipGetLastError();
+}
+
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
                                  const double *wp, int ns, int np, int n_fft, const float2 *tw,
-                                 void *q, int q_format, hipStream_t stream) {
+                                 void *q, int q_format, float *qscale, hipStream_t stream) {
     int logN = 0;
     while ((1 << logN) < n_fft) ++logN;
     const size_t lds = sizeof(float2) * (size_t)n_fft;
     if (q_format == SARBP_Q_C16) {
         static std::once_flag once;
-        std::call_once(once, [] {
-            (void)hipFuncSetAttribute((const void *)&range_profile_kernel<__half2>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
-        });
+        std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_kernel<__half2>); });
         hipLaunchKernelGGL(range_profile_kernel<__half2>, dim3(np), dim3(kFftThreads), lds, stream,
-                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (__half2 *)q);
+                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (__half2 *)q,
+                           qscale);
     } else {
         static std::once_flag once;
-        std::call_once(once, [] {
-            (void)hipFuncSetAttribute((const void *)&range_profile_kernel<float2>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
-        });
+        std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_kernel<float2>); });
         hipLaunchKernelGGL(range_profile_kernel<float2>, dim3(np), dim3(kFftThreads), lds, stream,
-                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (float2 *)q);
+                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (float2 *)q,
+                           (float *)nullptr);
     }
     return hipGetLastError();
 }

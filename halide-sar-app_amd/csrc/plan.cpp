@@ -15,6 +15,7 @@ struct sarbp_plan {
     double *ws = nullptr;       // Taylor window over samples, f64 [ns]
     double *wp = nullptr;       // Taylor window over pulses, f64 [np]
     void *q = nullptr;          // range profiles [np][N] in q_format
+    float *qscale = nullptr;    // per-profile inverse scale of f16 profiles [np]
     PulseConst *pc = nullptr;   // per-pulse constants [np]
     BpHeader *hdr = nullptr;    // launch-wide constants
     const float2 *tw = nullptr; // twiddles exp(-2 pi i t / N)
@@ -119,6 +120,7 @@ extern "C" int sarbp_plan_destroy(sarbp_plan *p) {
     (void)hipFree(p->ws);
     (void)hipFree(p->wp);
     (void)hipFree(p->q);
+    (void)hipFree(p->qscale);
     (void)hipFree(p->pc);
     (void)hipFree(p->hdr);
     delete p;
@@ -154,7 +156,9 @@ extern "C" int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out) {
               hipMalloc(&p->wp, sizeof(double) * np) == hipSuccess &&
               hipMalloc(&p->q, qelem * (size_t)np * (size_t)N) == hipSuccess &&
               hipMalloc(&p->pc, sizeof(PulseConst) * (size_t)np) == hipSuccess &&
-              hipMalloc(&p->hdr, sizeof(BpHeader)) == hipSuccess;
+              hipMalloc(&p->hdr, sizeof(BpHeader)) == hipSuccess &&
+              (d->q_format != SARBP_Q_C16 ||
+               hipMalloc(&p->qscale, sizeof(float) * (size_t)np) == hipSuccess);
     if (!ok) {
         sarbp_plan_destroy(p);
         set_error("device allocation failed for plan (np=%d, N=%d)", np, N);
@@ -178,6 +182,26 @@ extern "C" int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out) {
 
 extern "C" void *sarbp_plan_q_buffer(sarbp_plan *p) { return p ? p->q : nullptr; }
 
+extern "C" int sarbp_plan_read_pulse_norms(sarbp_plan *p, double *host_out, int32_t n) {
+    if (!p || !host_out) {
+        set_error("sarbp_plan_read_pulse_norms: NULL argument");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    if (n < 0 || n > p->d.npulses) {
+        set_error("sarbp_plan_read_pulse_norms: n (%d) outside [0, npulses]", n);
+        return halide_error_code_param_too_large;
+    }
+    DeviceGuard g(p->d.device);
+    std::vector<PulseConst> h((size_t)n);
+    if (n > 0 && hipMemcpy(h.data(), p->pc, sizeof(PulseConst) * (size_t)n,
+                           hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("sarbp_plan_read_pulse_norms: copy failed");
+        return halide_error_code_copy_to_host_failed;
+    }
+    for (int i = 0; i < n; ++i) host_out[i] = h[i].nr0;
+    return 0;
+}
+
 extern "C" int sarbp_plan_range_profiles(sarbp_plan *p, const void *phs_dev, const void *k_r_dev,
                                          void *q_dev, void *stream) {
     if (!p || !phs_dev || !k_r_dev) {
@@ -187,7 +211,8 @@ extern "C" int sarbp_plan_range_profiles(sarbp_plan *p, const void *phs_dev, con
     DeviceGuard g(p->d.device);
     hipError_t e = launch_range_profiles(phs_dev, (const float *)k_r_dev, p->ws, p->wp,
                                          p->d.nsamples, p->d.npulses, p->d.n_fft, p->tw,
-                                         q_dev ? q_dev : p->q, p->d.q_format, (hipStream_t)stream);
+                                         q_dev ? q_dev : p->q, p->d.q_format, p->qscale,
+                                         (hipStream_t)stream);
     if (e != hipSuccess) {
         set_error("range-profile launch failed: %s", hipGetErrorString(e));
         return halide_error_code_device_run_failed;
@@ -212,6 +237,7 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
         BpArgs a;
         a.q = q_dev ? q_dev : p->q;
         a.q_format = d.q_format;
+        a.qscale = d.q_format == SARBP_Q_C16 ? p->qscale : nullptr;
         a.n_fft = d.n_fft;
         a.np = d.npulses;
         a.pc = p->pc;

@@ -137,6 +137,9 @@ int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const void *k_r_dev,
                           const void *pos_dev, const void *r_dev, void *out_dev, void *stream);
 /* Device pointer of the plan's range-profile buffer. */
 void *sarbp_plan_q_buffer(sarbp_plan *p);
+/* Copy the first n entries of the plan's ||pos_p|| table (as computed by the last accumulate,
+ * under the plan's norm_r0 policy) to host memory; synchronous. */
+int sarbp_plan_read_pulse_norms(sarbp_plan *p, double *host_out, int32_t n);
 
 /* dB and u8 image output on the device for an f64 complex image of npix pixels. dB_dev and/or
  * u8_dev may be NULL. Synchronous with respect to `stream` only. */

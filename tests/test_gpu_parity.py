@@ -245,3 +245,28 @@ def test_q_f16_storage_error(sar, orc, pipe, torch_dev, scenes):
     e32, e16 = rel_l2(outs[0], g["img"]), rel_l2(outs[1], g["img"])
     print(f"C1 centre: Q c32 rel_l2 {e32:.3e}, Q c16 rel_l2 {e16:.3e}")
     assert e32 < TOL and e16 < 5e-3
+
+
+@pytest.mark.parametrize("policy", [0, 1, 2])
+def test_pulse_norm_table_bit_exact(sar, orc, scenes, torch_dev, policy):
+    """||pos_p|| (backprojection.cpp:108) decides the phase of every pulse: one f32 ulp at 15 km is
+    ~1 mm, ~0.4 rad at X band. The device table must equal the oracle's bit for bit under each
+    policy, for the benchmark geometries and random positions at 1-60 km."""
+    torch = torch_dev
+    rng = np.random.default_rng(policy + 40)
+    pos = [scenes.config_scene("C1").pos, scenes.config_scene("C2").pos,
+           (rng.standard_normal((4000, 3)) * rng.uniform(1e3, 6e4, (4000, 1))).astype(np.float32)]
+    pos = np.ascontiguousarray(np.concatenate(pos), np.float32)
+    n = pos.shape[0]
+    plan = sar.Plan(16, n, 32, 17, 0.5, 1, 1, norm_r0_policy=policy)
+    phs = torch.zeros((n, 16), dtype=torch.complex64, device="cuda")
+    k_r = torch.ones(16, dtype=torch.float32, device="cuda")
+    r = torch.zeros((3, 1), dtype=torch.float64, device="cuda")
+    out = torch.empty((1, 1), dtype=torch.complex128, device="cuda")
+    plan.execute(phs, k_r, torch.from_numpy(pos).cuda(), r, out)
+    torch.cuda.synchronize()
+    got = plan.pulse_norms()
+    _, ref = orc.norm_r0(pos, policy)
+    bad = np.count_nonzero(got != ref)
+    print(f"policy {policy}: {bad} of {n} norms differ")
+    assert bad == 0

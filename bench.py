@@ -44,6 +44,9 @@ def main():
                     help="image rows the CPU baseline computes (0 disables it)")
     ap.add_argument("--check-pixels", type=int, default=512)
     ap.add_argument("--q-format", default="c32", choices=["c32", "c16"])
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="after the steps, run one plain device-to-device copy of this many MiB "
+                         "(reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -88,6 +91,16 @@ def main():
     value = pp / (step_ms * 1e-3) / 1e9
     log(f"bench: step {step_ms:.3f} ms (range profiles {fft_ms:.3f} ms, accumulate {acc_ms:.3f} ms)"
         f" -> {value:.1f} Gpixel-pulses/s")
+
+    if args.copy_calibration_mb > 0:
+        n = args.copy_calibration_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=ds.dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: calibration copy of {args.copy_calibration_mb} MiB done")
+        del src, dst
 
     # dB + u8 output of the same image (reported beside the step, not part of it)
     npix = grid["nu"] * grid["nv"]

@@ -163,6 +163,7 @@ _lib.sarbp_plan_accumulate.argtypes = [_P, _P, _P, _P, _P, _P, _P]
 _lib.sarbp_plan_q_buffer.argtypes = [_P]
 _lib.sarbp_plan_q_buffer.restype = _P
 _lib.sarbp_plan_read_pulse_norms.argtypes = [_P, _P, _i32]
+_lib.sarbp_plan_read_profiles.argtypes = [_P, _P]
 _lib.sarbp_img_output_device.argtypes = [_P, _i64, _P, _P, _f64, _f64, _P]
 _lib.sarbp_sim_phs_device.argtypes = [_P, _i32, _P, _i32, _P, _P, _i32, _P, _P]
 
@@ -365,6 +366,14 @@ class Plan:
                                         _stream_handle(stream))
         if rv != 0:
             raise RuntimeError(f"sarbp_plan_accumulate failed ({rv}): {last_error()}")
+
+    def read_profiles(self) -> np.ndarray:
+        """The plan's range profiles Q as complex64 (npulses, n_fft) (synchronises the device)."""
+        out = np.empty((self.desc.npulses, self.desc.n_fft), np.complex64)
+        rv = _lib.sarbp_plan_read_profiles(self._p, out.ctypes.data_as(_P))
+        if rv != 0:
+            raise RuntimeError(f"sarbp_plan_read_profiles failed ({rv}): {last_error()}")
+        return out
 
     def pulse_norms(self, n=None) -> np.ndarray:
         """The ||pos_p|| table of the last accumulate (synchronous)."""

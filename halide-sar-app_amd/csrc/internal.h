@@ -77,5 +77,9 @@ const float2 *twiddles(int device, int n_fft, hipStream_t stream);
 void release_twiddles();
 
 constexpr int kMaxFft = 16384;
+// Range-profile rows are stored padded to N + kQPad elements (see the storage comment in kernels.hip).
+constexpr int kQPad = 4;
+// Bytes per stored range-profile element: (q, dq) in complex f32 or complex f16.
+inline size_t q_elem_bytes(int q_format) { return q_format == SARBP_Q_C16 ? 8 : 16; }
 
 }  // namespace sarbp

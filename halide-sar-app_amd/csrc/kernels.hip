@@ -28,26 +28,58 @@ __device__ inline float2 cmul(float2 a, float2 b) {
 
 __device__ inline int floordiv2(int a) { return (a >= 0) ? a / 2 : -((-a + 1) / 2); }
 
-template <typename QT>
-__device__ inline float2 loadq(const QT *p);
-template <>
-__device__ inline float2 loadq<float2>(const float2 *p) {
-    return *p;
-}
-template <>
-__device__ inline float2 loadq<__half2>(const __half2 *p) {
-    return __half22float2(*p);
-}
+// Range-profile storage: element j of a padded row holds q = Q[clamp(j - 1, 0, N - 1)] and
+// dq = Q[clamp(j, 0, N - 1)] - q, so that one 16-byte (8-byte for f16) gather yields both ends of the
+// interpolation interval and the clamped edges of the reference's interp (backprojection.cpp:59-62)
+// come out of the data: rows 0 and N + 1 have dq = 0.
+struct QPairF32 {
+    float qr, qi, dr, di;
+};
+struct QPairF16 {
+    __half2 q, dq;
+};
 
-template <typename QT>
-__device__ inline void storeq(QT *p, float2 v);
-template <>
-__device__ inline void storeq<float2>(float2 *p, float2 v) {
+__device__ inline void storeq(QPairF32 *p, float2 q, float2 dq) {
+    QPairF32 v;
+    v.qr = q.x;
+    v.qi = q.y;
+    v.dr = dq.x;
+    v.di = dq.y;
     *p = v;
 }
-template <>
-__device__ inline void storeq<__half2>(__half2 *p, float2 v) {
-    *p = __float22half2_rn(v);
+__device__ inline void storeq(QPairF16 *p, float2 q, float2 dq) {
+    QPairF16 v;
+    v.q = __float22half2_rn(q);
+    v.dq = __float22half2_rn(dq);
+    *p = v;
+}
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+
+// Buffer resource over one padded row (raw buffer, 32-bit data format word as for gfx9 raw loads):
+// reads at byte offsets past num_records return 0 instead of touching memory.
+__device__ inline __amdgpu_buffer_rsrc_t row_rsrc(const void *row, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(row), (short)0, (int)bytes,
+                                             0x00020000);
+}
+__device__ inline float4 loadq(const QPairF32 *, __amdgpu_buffer_rsrc_t r, unsigned idx) {
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, idx * 16u, 0, 0);
+    const f32x4 f = __builtin_bit_cast(f32x4, v);
+    return make_float4(f.x, f.y, f.z, f.w);
+}
+__device__ inline float4 loadq(const QPairF16 *, __amdgpu_buffer_rsrc_t r, unsigned idx) {
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8u, 0, 0);
+    const f16x4 h = __builtin_bit_cast(f16x4, v);
+    return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
+}
+
+// trunc(t) clamped to [0, hi]: v_cvt_i32_f64 saturates (NaN gives 0), v_med3_i32 clamps.
+__device__ inline int trunc_clamp(double t, int hi) {
+    int i;
+    asm("v_cvt_i32_f64 %0, %1\n\tv_med3_i32 %0, %0, 0, %2" : "=&v"(i) : "v"(t), "s"(hi));
+    return i;
 }
 
 // sqrt(s) from the hardware reciprocal square root (about 23 correct bits) and one
@@ -98,7 +130,7 @@ __global__ void pulse_const_kernel(const float *__restrict__ pos, const float *_
     c.py = y;
     c.pz = z;
     c.C = c.px * c.px + c.py * c.py + c.pz * c.pz;
-    c.B = (nr0 - lsa) * inv_lsr;
+    c.B = (nr0 - lsa) * inv_lsr + 1.0;  // +1: row element j holds Q[j - 1]
     c.G = -gamma * nr0;
     c.nr0 = nr0;
     c.pad = 0.0;
@@ -206,7 +238,7 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
         Ns = 2;
     }
     for (; Ns < N; Ns *= 4) fft_pass<4, kMaxFft / 4 / kFftThreads>(buf, N, Ns, tw, tid);
-    QT *dst = q + (size_t)p * N;
+    QT *dst = q + (size_t)p * (N + kQPad);
     float scale = 1.0f;
     if (qscale) {
         // f16 storage: scale the profile by 2^e so that its largest component is about 2^14
@@ -224,9 +256,13 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
         scale = ldexpf(1.0f, e);
         if (tid == 0) qscale[p] = ldexpf(1.0f, -e);
     }
-    for (int k = tid; k < N; k += kFftThreads) {
-        float2 v = buf[(k + half) & (N - 1)];
-        storeq<QT>(dst + k, make_float2(v.x * scale, v.y * scale));
+    // Q(k) = dft((k + N/2) mod N) (backprojection.cpp:104), written as padded (q, dq) pairs
+    for (int j = tid; j < N + kQPad; j += kFftThreads) {
+        const int k0 = min(max(j - 1, 0), N - 1), k1 = min(j, N - 1);
+        const float2 a = buf[(k0 + half) & (N - 1)], b = buf[(k1 + half) & (N - 1)];
+        const float2 q = make_float2(a.x * scale, a.y * scale);
+        const float2 e = make_float2(b.x * scale, b.y * scale);
+        storeq(dst + j, q, make_float2(e.x - q.x, e.y - q.y));
     }
 }
 
@@ -236,20 +272,24 @@ constexpr int kBpThreads = 256;
 constexpr int kBpPulseBlock = 64;
 
 // Tile of 16 x (16 * PPT) output pixels per block; thread (tx, ty) owns pixels (tx, ty + 16 j).
-// Each pixel keeps -2 r and |r|^2 in f64 registers; per pulse:
-//   d     = sqrt(|r|^2 + |pos|^2 - 2 r.pos)                  f64
-//   luts  = B_p - d / lsr ; i = floor(luts) ; w = luts - i   f64 (index), f32 (weight)
-//   q     = lerp(Q[clamp(i)], Q[clamp(i + 1)], w)            f32 complex (gathered from L1/L2)
-//   phase = fract(gamma * d + G_p)                           f64 -> f32 revolutions
-//   acc  += q * (cos, sin)(2 pi phase)                       f32, flushed to f64 every 64 pulses
+// Each pixel keeps -2 r and |r|^2 in f64 registers; per pulse p (constants from the pulse table):
+//   d     = sqrt(|r|^2 + |pos_p|^2 - 2 r.pos_p)                     f64, rsq + one Newton step
+//   t     = B_p - d / lsr      (B_p includes +1 for the padded row)  f64
+//   i     = clamp(trunc(t), 0, N + 1);  w = fract(t)                int, f64 -> f32
+//   q     = row_p[i].q + w * row_p[i].dq                            one 16-byte gather, f32
+//   phase = fract(gamma * d + G_p)                                   f64 revolutions -> f32
+//   acc  += q * (cos, sin)(2 pi phase)                               f32, flushed to f64 per 64 pulses
+// For t in [-1, 0) trunc gives 0 and row 0 has dq = 0; for t < -1 or t >= N the clamp lands on rows
+// 0 or N + 1 whose dq is 0 — the reference's clamped indices with any weight (backprojection.cpp:59-64).
 template <typename QT, int PPT>
 __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int bx = blockIdx.x * 16, by = blockIdx.y * (16 * PPT);
-    const double inv_lsr = a.hdr->inv_lsr;
+    const double ninv = -a.hdr->inv_lsr;
     const double gamma = a.hdr->gamma;
     const int N = a.n_fft;
     const int np = a.np;
+    const int ihi = N + 1;
 
     double m2x[PPT], m2y[PPT], m2z[PPT], r2[PPT];
     bool ok[PPT];
@@ -277,7 +317,10 @@ __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
 #pragma unroll
     for (int j = 0; j < PPT; ++j) accr[j] = acci[j] = 0.0;
 
+    constexpr bool kScaled = sizeof(QT) == sizeof(QPairF16);
     const QT *qbase = reinterpret_cast<const QT *>(a.q);
+    const size_t row_elems = (size_t)N + kQPad;
+    const unsigned row_bytes = (unsigned)(row_elems * sizeof(QT));
     for (int p0 = 0; p0 < np; p0 += kBpPulseBlock) {
         float sr[PPT], si[PPT];
 #pragma unroll
@@ -285,31 +328,33 @@ __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
         const int pe = min(np, p0 + kBpPulseBlock);
         for (int p = p0; p < pe; ++p) {
             const PulseConst c = a.pc[p];
-            const QT *row = qbase + (size_t)p * N;
-            const float qs = a.qscale ? a.qscale[p] : 1.0f;
+            const __amdgpu_buffer_rsrc_t row = row_rsrc(qbase + (size_t)p * row_elems, row_bytes);
+            const float qs = kScaled ? a.qscale[p] : 1.0f;
+            int ii[PPT];
+            float ww[PPT], ff[PPT];
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
                 const double s2 = __builtin_fma(
                     m2x[j], c.px, __builtin_fma(m2y[j], c.py, __builtin_fma(m2z[j], c.pz, r2[j] + c.C)));
                 const double d = fast_sqrt(s2);
-                const double luts = __builtin_fma(-inv_lsr, d, c.B);
-                const double fl = __builtin_floor(luts);
-                const int i = (int)fl;
-                const float wt = (float)(luts - fl);
-                const int i0 = min(max(i, 0), N - 1);
-                const int i1 = min(max(i + 1, 0), N - 1);
-                const float2 q0 = loadq<QT>(row + i0);
-                const float2 q1 = loadq<QT>(row + i1);
-                float qr = __builtin_fmaf(wt, q1.x - q0.x, q0.x);
-                float qi = __builtin_fmaf(wt, q1.y - q0.y, q0.y);
-                if (a.qscale) {
+                const double t = __builtin_fma(ninv, d, c.B);
+                ii[j] = trunc_clamp(t, ihi);
+                ww[j] = (float)__builtin_amdgcn_fract(t);
+                ff[j] = (float)__builtin_amdgcn_fract(__builtin_fma(gamma, d, c.G));
+            }
+            float4 e[PPT];
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) e[j] = loadq(qbase, row, (unsigned)ii[j]);
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) {
+                const float sn = __builtin_amdgcn_sinf(ff[j]);
+                const float cs = __builtin_amdgcn_cosf(ff[j]);
+                float qr = __builtin_fmaf(ww[j], e[j].z, e[j].x);
+                float qi = __builtin_fmaf(ww[j], e[j].w, e[j].y);
+                if (kScaled) {
                     qr *= qs;
                     qi *= qs;
                 }
-                const double ph = __builtin_fma(gamma, d, c.G);
-                const float f = (float)__builtin_amdgcn_fract(ph);
-                const float sn = __builtin_amdgcn_sinf(f);
-                const float cs = __builtin_amdgcn_cosf(f);
                 sr[j] = __builtin_fmaf(qr, cs, sr[j]);
                 sr[j] = __builtin_fmaf(-qi, sn, sr[j]);
                 si[j] = __builtin_fmaf(qr, sn, si[j]);
@@ -473,15 +518,15 @@ hipError_t launch_range_profiles(const void *phs, const float *k_r, const double
     const size_t lds = sizeof(float2) * (size_t)n_fft;
     if (q_format == SARBP_Q_C16) {
         static std::once_flag once;
-        std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_kernel<__half2>); });
-        hipLaunchKernelGGL(range_profile_kernel<__half2>, dim3(np), dim3(kFftThreads), lds, stream,
-                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (__half2 *)q,
+        std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_kernel<QPairF16>); });
+        hipLaunchKernelGGL(range_profile_kernel<QPairF16>, dim3(np), dim3(kFftThreads), lds, stream,
+                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (QPairF16 *)q,
                            qscale);
     } else {
         static std::once_flag once;
-        std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_kernel<float2>); });
-        hipLaunchKernelGGL(range_profile_kernel<float2>, dim3(np), dim3(kFftThreads), lds, stream,
-                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (float2 *)q,
+        std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_kernel<QPairF32>); });
+        hipLaunchKernelGGL(range_profile_kernel<QPairF32>, dim3(np), dim3(kFftThreads), lds, stream,
+                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (QPairF32 *)q,
                            (float *)nullptr);
     }
     return hipGetLastError();
@@ -492,9 +537,9 @@ hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream) {
     const dim3 grid((a.width + 15) / 16, (a.height + 16 * PPT - 1) / (16 * PPT));
     if (a.width <= 0 || a.height <= 0) return hipSuccess;
     if (a.q_format == SARBP_Q_C16)
-        hipLaunchKernelGGL((accumulate_kernel<__half2, PPT>), grid, dim3(kBpThreads), 0, stream, a);
+        hipLaunchKernelGGL((accumulate_kernel<QPairF16, PPT>), grid, dim3(kBpThreads), 0, stream, a);
     else
-        hipLaunchKernelGGL((accumulate_kernel<float2, PPT>), grid, dim3(kBpThreads), 0, stream, a);
+        hipLaunchKernelGGL((accumulate_kernel<QPairF32, PPT>), grid, dim3(kBpThreads), 0, stream, a);
     return hipGetLastError();
 }
 

@@ -151,10 +151,10 @@ extern "C" int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out) {
     std::vector<double> ws((size_t)ns), wp((size_t)np);
     taylor_window(ns, d->taylor_s_l, d->taylor_den_policy, ws.data());
     taylor_window(np, d->taylor_s_l, d->taylor_den_policy, wp.data());
-    const size_t qelem = d->q_format == SARBP_Q_C16 ? 4 : 8;
+    const size_t qelem = q_elem_bytes(d->q_format);
     bool ok = hipMalloc(&p->ws, sizeof(double) * ns) == hipSuccess &&
               hipMalloc(&p->wp, sizeof(double) * np) == hipSuccess &&
-              hipMalloc(&p->q, qelem * (size_t)np * (size_t)N) == hipSuccess &&
+              hipMalloc(&p->q, qelem * (size_t)np * (size_t)(N + kQPad)) == hipSuccess &&
               hipMalloc(&p->pc, sizeof(PulseConst) * (size_t)np) == hipSuccess &&
               hipMalloc(&p->hdr, sizeof(BpHeader)) == hipSuccess &&
               (d->q_format != SARBP_Q_C16 ||
@@ -181,6 +181,74 @@ extern "C" int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out) {
 }
 
 extern "C" void *sarbp_plan_q_buffer(sarbp_plan *p) { return p ? p->q : nullptr; }
+
+namespace {
+float half_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h >> 15) << 31;
+    int e = (h >> 10) & 0x1f;
+    uint32_t m = h & 0x3ff;
+    uint32_t bits;
+    if (e == 0x1f) {
+        bits = sign | 0x7f800000u | (m << 13);
+    } else if (e == 0) {
+        if (m == 0) {
+            bits = sign;
+        } else {  // subnormal: renormalise
+            int sh = 0;
+            while (!(m & 0x400)) {
+                m <<= 1;
+                ++sh;
+            }
+            m &= 0x3ff;
+            bits = sign | ((uint32_t)(127 - 15 + 1 - sh) << 23) | (m << 13);
+        }
+    } else {
+        bits = sign | ((uint32_t)(e - 15 + 127) << 23) | (m << 13);
+    }
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+}  // namespace
+
+extern "C" int sarbp_plan_read_profiles(sarbp_plan *p, float *host_out) {
+    if (!p || !host_out) {
+        set_error("sarbp_plan_read_profiles: NULL argument");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    DeviceGuard g(p->d.device);
+    const int np = p->d.npulses, N = p->d.n_fft;
+    const size_t eb = q_elem_bytes(p->d.q_format);
+    const size_t row = (size_t)(N + kQPad) * eb;
+    std::vector<uint8_t> h(row * (size_t)np);
+    std::vector<float> sc((size_t)np, 1.0f);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(h.data(), p->q, h.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        (p->qscale && hipMemcpy(sc.data(), p->qscale, sizeof(float) * np,
+                                hipMemcpyDeviceToHost) != hipSuccess)) {
+        set_error("sarbp_plan_read_profiles: copy failed");
+        return halide_error_code_copy_to_host_failed;
+    }
+    for (int pp = 0; pp < np; ++pp) {
+        for (int k = 0; k < N; ++k) {
+            const uint8_t *e = h.data() + (size_t)pp * row + (size_t)(k + 1) * eb;
+            float re, im;
+            if (p->d.q_format == SARBP_Q_C16) {
+                uint16_t a, b;
+                std::memcpy(&a, e, 2);
+                std::memcpy(&b, e + 2, 2);
+                re = half_to_float(a) * sc[pp];
+                im = half_to_float(b) * sc[pp];
+            } else {
+                std::memcpy(&re, e, 4);
+                std::memcpy(&im, e + 4, 4);
+            }
+            host_out[2 * ((size_t)pp * N + k)] = re;
+            host_out[2 * ((size_t)pp * N + k) + 1] = im;
+        }
+    }
+    return 0;
+}
 
 extern "C" int sarbp_plan_read_pulse_norms(sarbp_plan *p, double *host_out, int32_t n) {
     if (!p || !host_out) {

@@ -130,13 +130,17 @@ int sarbp_plan_execute(sarbp_plan *p, const void *phs_dev, const void *k_r_dev,
                        const void *pos_dev, const void *r_dev, void *out_dev, void *stream);
 
 /* The two stages separately (for profiling and tests). q_dev may be NULL to use the plan's own
- * range-profile buffer; otherwise it must hold npulses * n_fft complex values in q_format. */
+ * range-profile buffer; otherwise it must hold npulses * (n_fft + 4) stored elements of
+ * 16 bytes (SARBP_Q_C32) or 8 bytes (SARBP_Q_C16): the padded (q, dq) rows the kernels use. */
 int sarbp_plan_range_profiles(sarbp_plan *p, const void *phs_dev, const void *k_r_dev,
                               void *q_dev, void *stream);
 int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const void *k_r_dev,
                           const void *pos_dev, const void *r_dev, void *out_dev, void *stream);
 /* Device pointer of the plan's range-profile buffer. */
 void *sarbp_plan_q_buffer(sarbp_plan *p);
+/* Copy the plan's range profiles Q to host memory as complex f32 [npulses][n_fft] (undoing the
+ * padded storage and, for f16, the per-profile scale); synchronises the device. */
+int sarbp_plan_read_profiles(sarbp_plan *p, float *host_out);
 /* Copy the first n entries of the plan's ||pos_p|| table (as computed by the last accumulate,
  * under the plan's norm_r0 policy) to host memory; synchronous. */
 int sarbp_plan_read_pulse_norms(sarbp_plan *p, double *host_out, int32_t n);

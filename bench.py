@@ -44,6 +44,8 @@ def main():
                     help="image rows the CPU baseline computes (0 disables it)")
     ap.add_argument("--check-pixels", type=int, default=512)
     ap.add_argument("--q-format", default="c32", choices=["c32", "c16"])
+    ap.add_argument("--tile-ppt", type=int, default=0,
+                    help="pixels per thread of the accumulation tile (0: library default)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the steps, run one plain device-to-device copy of this many MiB "
                          "(reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -65,7 +67,7 @@ def main():
     log(f"bench: {args.config} np={sc.npulses} ns={sc.nsamples} image {grid['nu']}x{grid['nv']}")
     ds = pipe.DeviceScene(sc, grid)
     qf = sar.Q_C16 if args.q_format == "c16" else sar.Q_C32
-    plan = ds.plan(q_format=qf)
+    plan = ds.plan(q_format=qf, tile_ppt=args.tile_ppt)
     out = ds.output()
     stream = torch.cuda.current_stream()
     pp = pipe.pixel_pulses(sc, grid)

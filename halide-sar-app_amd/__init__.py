@@ -150,7 +150,8 @@ class PlanDesc(ctypes.Structure):
                 ("delta_r", _f64), ("nu", _i32), ("nv", _i32), ("x0", _i32), ("y0", _i32),
                 ("width", _i32), ("height", _i32), ("r_min", _i64), ("r_extent", _i64),
                 ("norm_r0_policy", _i32), ("taylor_den_policy", _i32), ("q_format", _i32),
-                ("range_mode", _i32), ("device", _i32), ("reserved", _i32 * 7)]
+                ("range_mode", _i32), ("device", _i32), ("tile_ppt", _i32),
+                ("reserved", _i32 * 6)]
 
 
 _lib.sarbp_plan_desc_init.argtypes = [ctypes.POINTER(PlanDesc)]
@@ -322,7 +323,7 @@ class Plan:
 
     def __init__(self, nsamples, npulses, n_fft, taylor_s_l, delta_r, nu, nv, region=None,
                  r_min=0, r_extent=None, norm_r0_policy=None, taylor_den_policy=None,
-                 q_format=Q_C32, range_mode=RANGE_F64, device=0):
+                 q_format=Q_C32, range_mode=RANGE_F64, device=0, tile_ppt=0):
         d = PlanDesc()
         _lib.sarbp_plan_desc_init(ctypes.byref(d))
         d.nsamples, d.npulses, d.n_fft, d.taylor_s_l = nsamples, npulses, n_fft, taylor_s_l
@@ -337,6 +338,7 @@ class Plan:
         if taylor_den_policy is not None:
             d.taylor_den_policy = taylor_den_policy
         d.q_format, d.range_mode, d.device = q_format, range_mode, device
+        d.tile_ppt = tile_ppt
         self.desc = d
         self._p = _P()
         rv = _lib.sarbp_plan_create(ctypes.byref(d), ctypes.byref(self._p))

@@ -34,13 +34,22 @@ void set_error(const char *fmt, ...) {
 const char *last_error() { return g_last_error.c_str(); }
 
 // backprojection.cpp:120 and :53 — the interpolation axis is linspace(floor(-ns dr / 2),
-// floor(ns dr / 2), N).
-RangeAxis range_axis(int ns, int n_fft, double delta_r) {
-    RangeAxis a;
+// floor(ns dr / 2), N); rows extend it by N / 4 elements on each side.
+RowLayout row_layout(int ns, int n_fft, double delta_r) {
+    RowLayout a;
+    a.n_fft = n_fft;
     a.lsa = std::floor((double)(-ns) * delta_r / 2.0);
     a.lsb = std::floor((double)ns * delta_r / 2.0);
     a.lsr = (a.lsb - a.lsa) / (double)(n_fft - 1);
     a.inv_lsr = 1.0 / a.lsr;
+    a.ext = n_fft / 4 > 2 ? n_fft / 4 : 2;
+    a.len = n_fft + 2 * a.ext + 2;
+    // |dr| = | ||pos|| - |r - pos| | <= |r| (+ the f32 rounding of ||pos||, < 1 % of a metre here);
+    // keep 2 elements of margin at each end of the row.
+    const double lo = -(a.lsa + (2 - a.ext) * a.lsr);               // dr >= lsa + (2 - ext) lsr
+    const double hi = a.lsa + (double)(n_fft - 1 + a.ext - 2) * a.lsr;  // t <= len - 3
+    a.r_cover = (lo < hi ? lo : hi) - 0.01;
+    if (a.r_cover < 0.0) a.r_cover = 0.0;
     return a;
 }
 

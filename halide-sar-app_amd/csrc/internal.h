@@ -11,31 +11,41 @@
 namespace sarbp {
 
 // Per-pulse constants of the accumulation, computed on the device from pos and k_r.
-// With d = |r - pos_p| (f64):   luts  = B - inv_lsr * d        (linspace index, f64)
-//                               phase = fract(gamma * d + G)    (revolutions of -k_c * dr)
-// and |r - pos_p|^2 = |r|^2 + C - 2 r.pos_p.
+// With d = |r - pos_p| (f64) the row coordinate of the interpolation is t = B - inv_lsr * d, and
+// |r - pos_p|^2 = |r|^2 + C - 2 r.pos_p.
 struct PulseConst {
     double px, py, pz;  // platform position, f32 widened
     double C;           // |pos_p|^2
-    double B;           // (nr0 - lsa) * inv_lsr
-    double G;           // -gamma * nr0
+    double B;           // (nr0 - lsa) * inv_lsr + ext
     double nr0;         // ||pos_p|| under the selected policy, widened
-    double pad;
+    double pad0, pad1;
 };
 
 // Launch-wide constants written next to the pulse table.
 struct BpHeader {
-    double k_c;      // k_r[nsamples / 2] (f32 widened)
-    double gamma;    // k_c / (2 pi)
-    double lsa;      // floor(-ns * delta_r / 2)
-    double inv_lsr;  // (N - 1) / (lsb - lsa)
+    double k_c;       // k_r[nsamples / 2] (f32 widened)
+    double gamma;     // k_c / (2 pi): revolutions of phase per metre of dr
+    double lsa;       // floor(-ns * delta_r / 2)
+    double inv_lsr;   // (N - 1) / (lsb - lsa)
+    double kappa;     // -gamma * lsr: revolutions of phase per unit of t
+    double g_ext;     // -gamma * (lsa - lsr * ext): phase at t = 0, revolutions
 };
 
-// Range-axis constants shared by the host and the device.
-struct RangeAxis {
+// Range profiles are stored per pulse as rows of `len` elements. Element j is the interpolation
+// interval starting at t = j, where t = luts + ext and luts is the reference's index on the
+// linspace axis (backprojection.cpp:53-57):
+//   A_j = Q[clamp(j - ext)] * R_j,  B_j = (Q[clamp(j - ext + 1)] - Q[clamp(j - ext)]) * R_j,
+//   R_j = exp(2 pi i fract(g_ext + kappa j)),  clamp to [0, N - 1].
+// The pixel's term q(t) * expj(-k_c dr) is then (A_j + w B_j) * exp(2 pi i kappa w) with
+// j = floor(t), w = t - j: the phase of the reference depends on dr only, and so on t only.
+// `ext` edge-replicated elements on each side make the reference's clamped interpolation
+// (backprojection.cpp:59-62) exact for |dr| up to `r_cover`.
+struct RowLayout {
+    int n_fft, ext, len;
     double lsa, lsb, lsr, inv_lsr;
+    double r_cover;  // pixels with |r| <= r_cover have every t in [0, len - 1)
 };
-RangeAxis range_axis(int ns, int n_fft, double delta_r);
+RowLayout row_layout(int ns, int n_fft, double delta_r);
 
 // Thread-local last-error string.
 void set_error(const char *fmt, ...);
@@ -44,12 +54,13 @@ void set_error(const char *fmt, ...);
 int taylor_window(int num, int s_l, int den_policy, double *out);
 
 // Kernel launchers (kernels.hip). All are asynchronous on `stream`; return hipError_t.
-hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np, int n_fft,
-                               double delta_r, int norm_policy, PulseConst *pc, BpHeader *hdr,
-                               hipStream_t stream);
+hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np,
+                               const RowLayout &lay, int norm_policy, PulseConst *pc,
+                               BpHeader *hdr, hipStream_t stream);
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
-                                 const double *wp, int ns, int np, int n_fft, const float2 *tw,
-                                 void *q, int q_format, float *qscale, hipStream_t stream);
+                                 const double *wp, int ns, int np, const RowLayout &lay,
+                                 const float2 *tw, float2 *rot, void *q, int q_format,
+                                 float *qscale, hipStream_t stream);
 struct BpArgs {
     const void *q;
     int q_format;
@@ -57,7 +68,10 @@ struct BpArgs {
     int n_fft, np;
     const PulseConst *pc;
     const BpHeader *hdr;
+    int ext, len;
+    double r_cover2;
     const double *r;
+    int ppt;  // pixels per thread (tile height 16 * ppt)
     int64_t r_min, r_extent;
     int nu, nv, x0, y0, width, height;
     double2 *out;
@@ -77,8 +91,6 @@ const float2 *twiddles(int device, int n_fft, hipStream_t stream);
 void release_twiddles();
 
 constexpr int kMaxFft = 16384;
-// Range-profile rows are stored padded to N + kQPad elements (see the storage comment in kernels.hip).
-constexpr int kQPad = 4;
 // Bytes per stored range-profile element: (q, dq) in complex f32 or complex f16.
 inline size_t q_elem_bytes(int q_format) { return q_format == SARBP_Q_C16 ? 8 : 16; }
 

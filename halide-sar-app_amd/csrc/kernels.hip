@@ -28,10 +28,7 @@ __device__ inline float2 cmul(float2 a, float2 b) {
 
 __device__ inline int floordiv2(int a) { return (a >= 0) ? a / 2 : -((-a + 1) / 2); }
 
-// Range-profile storage: element j of a padded row holds q = Q[clamp(j - 1, 0, N - 1)] and
-// dq = Q[clamp(j, 0, N - 1)] - q, so that one 16-byte (8-byte for f16) gather yields both ends of the
-// interpolation interval and the clamped edges of the reference's interp (backprojection.cpp:59-62)
-// come out of the data: rows 0 and N + 1 have dq = 0.
+// Range-profile storage: rows of RowLayout::len elements (A_j, B_j), see RowLayout in internal.h.
 struct QPairF32 {
     float qr, qi, dr, di;
 };
@@ -75,6 +72,13 @@ __device__ inline float4 loadq(const QPairF16 *, __amdgpu_buffer_rsrc_t r, unsig
     return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
 }
 
+// trunc(t): v_cvt_i32_f64 saturates out-of-range values and gives 0 for NaN.
+__device__ inline int trunc_i32(double t) {
+    int i;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(i) : "v"(t));
+    return i;
+}
+
 // trunc(t) clamped to [0, hi]: v_cvt_i32_f64 saturates (NaN gives 0), v_med3_i32 clamps.
 __device__ inline int trunc_clamp(double t, int hi) {
     int i;
@@ -97,18 +101,21 @@ __device__ inline double fast_sqrt(double s) {
 // ------------------------------------------------------------------ per-pulse constants
 
 __global__ void pulse_const_kernel(const float *__restrict__ pos, const float *__restrict__ k_r,
-                                   int ns, int np, double lsa, double inv_lsr, int policy,
-                                   PulseConst *__restrict__ pc, BpHeader *__restrict__ hdr) {
+                                   int ns, int np, double lsa, double lsr, double inv_lsr, int ext,
+                                   int policy, PulseConst *__restrict__ pc,
+                                   BpHeader *__restrict__ hdr) {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    // k_c = k_r(nsamples / 2), f32 (backprojection.cpp:123)
-    const double k_c = (double)k_r[ns / 2];
-    const double gamma = k_c / (2.0 * M_PI);
     if (p == 0) {
+        // k_c = k_r(nsamples / 2), f32 (backprojection.cpp:123); phase = -k_c dr in revolutions
+        const double k_c = (double)k_r[ns / 2];
+        const double gamma = k_c / (2.0 * M_PI);
         hdr->k_c = k_c;
         hdr->gamma = gamma;
         hdr->lsa = lsa;
         hdr->inv_lsr = inv_lsr;
+        hdr->kappa = -gamma * lsr;
+        hdr->g_ext = -gamma * (lsa - lsr * (double)ext);
     }
     if (p >= np) return;
     const float x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
@@ -130,11 +137,26 @@ __global__ void pulse_const_kernel(const float *__restrict__ pos, const float *_
     c.py = y;
     c.pz = z;
     c.C = c.px * c.px + c.py * c.py + c.pz * c.pz;
-    c.B = (nr0 - lsa) * inv_lsr + 1.0;  // +1: row element j holds Q[j - 1]
-    c.G = -gamma * nr0;
+    // t = luts + ext with luts = (nr0 - d - lsa) / lsr (backprojection.cpp:53-54,117,120)
+    c.B = (nr0 - lsa) * inv_lsr + (double)ext;
     c.nr0 = nr0;
-    c.pad = 0.0;
+    c.pad0 = c.pad1 = 0.0;
     pc[p] = c;
+}
+
+// R_j = exp(2 pi i fract(g_ext + kappa j)) for the row elements, from k_c on the device.
+__global__ void rotation_kernel(const float *__restrict__ k_r, int ns, double lsa, double lsr,
+                                int ext, int len, float2 *__restrict__ rot) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= len) return;
+    const double gamma = (double)k_r[ns / 2] / (2.0 * M_PI);
+    const double g_ext = -gamma * (lsa - lsr * (double)ext);
+    const double kappa = -gamma * lsr;
+    const double ph = g_ext + kappa * (double)j;
+    const double f = ph - floor(ph);
+    double sn, cs;
+    sincospi(2.0 * f, &sn, &cs);
+    rot[j] = make_float2((float)cs, (float)sn);
 }
 
 // ------------------------------------------------------------------ range profiles + FFT
@@ -206,7 +228,8 @@ template <typename QT>
 __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
     const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
     const double *__restrict__ wp, int ns, int np, int N, int logN,
-    const float2 *__restrict__ tw, QT *__restrict__ q, float *__restrict__ qscale) {
+    const float2 *__restrict__ tw, const float2 *__restrict__ rot, int ext, int len,
+    QT *__restrict__ q, float *__restrict__ qscale) {
     extern __shared__ float2 buf[];
     __shared__ float red[kFftThreads / 64];
     const int p = blockIdx.x;
@@ -238,7 +261,7 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
         Ns = 2;
     }
     for (; Ns < N; Ns *= 4) fft_pass<4, kMaxFft / 4 / kFftThreads>(buf, N, Ns, tw, tid);
-    QT *dst = q + (size_t)p * (N + kQPad);
+    QT *dst = q + (size_t)p * len;
     float scale = 1.0f;
     if (qscale) {
         // f16 storage: scale the profile by 2^e so that its largest component is about 2^14
@@ -256,13 +279,14 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
         scale = ldexpf(1.0f, e);
         if (tid == 0) qscale[p] = ldexpf(1.0f, -e);
     }
-    // Q(k) = dft((k + N/2) mod N) (backprojection.cpp:104), written as padded (q, dq) pairs
-    for (int j = tid; j < N + kQPad; j += kFftThreads) {
-        const int k0 = min(max(j - 1, 0), N - 1), k1 = min(j, N - 1);
+    // Q(k) = dft((k + N/2) mod N) (backprojection.cpp:104), written as rotated (A, B) elements
+    for (int j = tid; j < len; j += kFftThreads) {
+        const int k0 = min(max(j - ext, 0), N - 1), k1 = min(max(j - ext + 1, 0), N - 1);
         const float2 a = buf[(k0 + half) & (N - 1)], b = buf[(k1 + half) & (N - 1)];
-        const float2 q = make_float2(a.x * scale, a.y * scale);
-        const float2 e = make_float2(b.x * scale, b.y * scale);
-        storeq(dst + j, q, make_float2(e.x - q.x, e.y - q.y));
+        const float2 q0 = make_float2(a.x * scale, a.y * scale);
+        const float2 dq = make_float2(b.x * scale - q0.x, b.y * scale - q0.y);
+        const float2 r = rot[j];
+        storeq(dst + j, cmul(q0, r), cmul(dq, r));
     }
 }
 
@@ -270,57 +294,31 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
 
 constexpr int kBpThreads = 256;
 constexpr int kBpPulseBlock = 64;
+constexpr int kDefaultPpt = 2;
 
 // Tile of 16 x (16 * PPT) output pixels per block; thread (tx, ty) owns pixels (tx, ty + 16 j).
 // Each pixel keeps -2 r and |r|^2 in f64 registers; per pulse p (constants from the pulse table):
-//   d     = sqrt(|r|^2 + |pos_p|^2 - 2 r.pos_p)                     f64, rsq + one Newton step
-//   t     = B_p - d / lsr      (B_p includes +1 for the padded row)  f64
-//   i     = clamp(trunc(t), 0, N + 1);  w = fract(t)                int, f64 -> f32
-//   q     = row_p[i].q + w * row_p[i].dq                            one 16-byte gather, f32
-//   phase = fract(gamma * d + G_p)                                   f64 revolutions -> f32
-//   acc  += q * (cos, sin)(2 pi phase)                               f32, flushed to f64 per 64 pulses
-// For t in [-1, 0) trunc gives 0 and row 0 has dq = 0; for t < -1 or t >= N the clamp lands on rows
-// 0 or N + 1 whose dq is 0 — the reference's clamped indices with any weight (backprojection.cpp:59-64).
-template <typename QT, int PPT>
-__global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int bx = blockIdx.x * 16, by = blockIdx.y * (16 * PPT);
-    const double ninv = -a.hdr->inv_lsr;
-    const double gamma = a.hdr->gamma;
-    const int N = a.n_fft;
-    const int np = a.np;
-    const int ihi = N + 1;
-
-    double m2x[PPT], m2y[PPT], m2z[PPT], r2[PPT];
-    bool ok[PPT];
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-        const int lx = bx + tx, ly = by + ty + 16 * j;
-        ok[j] = lx < a.width && ly < a.height;
-        const int x = a.x0 + lx, y = a.y0 + ly;
-        // xy_to_pixel = nu * (nv - y - 1) + x (backprojection.cpp:130); r is constant_exterior 0
-        const int64_t px = (int64_t)a.nu * (int64_t)(a.nv - y - 1) + x;
-        const int64_t idx = px - a.r_min;
-        double rx = 0.0, ry = 0.0, rz = 0.0;
-        if (ok[j] && idx >= 0 && idx < a.r_extent) {
-            rx = a.r[idx];
-            ry = a.r[a.r_extent + idx];
-            rz = a.r[2 * a.r_extent + idx];
-        }
-        m2x[j] = -2.0 * rx;
-        m2y[j] = -2.0 * ry;
-        m2z[j] = -2.0 * rz;
-        r2[j] = rx * rx + ry * ry + rz * rz;
-    }
-
-    double accr[PPT], acci[PPT];
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) accr[j] = acci[j] = 0.0;
-
+//   d  = sqrt(|r|^2 + |pos_p|^2 - 2 r.pos_p)          f64: v_rsq_f64 + one Newton step
+//   t  = B_p - d / lsr                                f64 row coordinate
+//   j  = floor(t), w = t - j                          int, f64 -> f32
+//   term = (A_j + w B_j) * exp(2 pi i kappa w)        one 16-byte gather; f32
+//   acc += term                                       f32, flushed to f64 every 64 pulses
+// This fast path needs every t of the block inside the row; a block whose pixels reach beyond the
+// row's cover radius takes the exact path instead: j clamped to the row, the phase
+// kappa (t - j) reduced in f64 (the clamped edge elements have B = 0, so the weight is moot).
+template <typename QT, int PPT, bool FAST>
+__device__ inline void accumulate_pulses(const BpArgs &a, const double (&m2x)[PPT],
+                                         const double (&m2y)[PPT], const double (&m2z)[PPT],
+                                         const double (&r2)[PPT], double (&accr)[PPT],
+                                         double (&acci)[PPT]) {
     constexpr bool kScaled = sizeof(QT) == sizeof(QPairF16);
+    const double ninv = -a.hdr->inv_lsr;
+    const double kappa = a.hdr->kappa;
+    const float kappa_f = (float)kappa;
+    const int np = a.np;
+    const int jhi = a.len - 2;
     const QT *qbase = reinterpret_cast<const QT *>(a.q);
-    const size_t row_elems = (size_t)N + kQPad;
-    const unsigned row_bytes = (unsigned)(row_elems * sizeof(QT));
+    const unsigned row_bytes = (unsigned)((size_t)a.len * sizeof(QT));
     for (int p0 = 0; p0 < np; p0 += kBpPulseBlock) {
         float sr[PPT], si[PPT];
 #pragma unroll
@@ -328,27 +326,34 @@ __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
         const int pe = min(np, p0 + kBpPulseBlock);
         for (int p = p0; p < pe; ++p) {
             const PulseConst c = a.pc[p];
-            const __amdgpu_buffer_rsrc_t row = row_rsrc(qbase + (size_t)p * row_elems, row_bytes);
+            const __amdgpu_buffer_rsrc_t row = row_rsrc(qbase + (size_t)p * a.len, row_bytes);
             const float qs = kScaled ? a.qscale[p] : 1.0f;
             int ii[PPT];
-            float ww[PPT], ff[PPT];
+            float ww[PPT], xx[PPT];
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
                 const double s2 = __builtin_fma(
                     m2x[j], c.px, __builtin_fma(m2y[j], c.py, __builtin_fma(m2z[j], c.pz, r2[j] + c.C)));
                 const double d = fast_sqrt(s2);
                 const double t = __builtin_fma(ninv, d, c.B);
-                ii[j] = trunc_clamp(t, ihi);
-                ww[j] = (float)__builtin_amdgcn_fract(t);
-                ff[j] = (float)__builtin_amdgcn_fract(__builtin_fma(gamma, d, c.G));
+                if (FAST) {
+                    ii[j] = trunc_i32(t);
+                    ww[j] = (float)__builtin_amdgcn_fract(t);
+                    xx[j] = kappa_f * ww[j];
+                } else {
+                    ii[j] = trunc_clamp(t, jhi);
+                    const double w = t - (double)ii[j];
+                    ww[j] = (float)w;
+                    xx[j] = (float)__builtin_amdgcn_fract(kappa * w);
+                }
             }
             float4 e[PPT];
 #pragma unroll
             for (int j = 0; j < PPT; ++j) e[j] = loadq(qbase, row, (unsigned)ii[j]);
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
-                const float sn = __builtin_amdgcn_sinf(ff[j]);
-                const float cs = __builtin_amdgcn_cosf(ff[j]);
+                const float sn = __builtin_amdgcn_sinf(xx[j]);
+                const float cs = __builtin_amdgcn_cosf(xx[j]);
                 float qr = __builtin_fmaf(ww[j], e[j].z, e[j].x);
                 float qi = __builtin_fmaf(ww[j], e[j].w, e[j].y);
                 if (kScaled) {
@@ -367,9 +372,52 @@ __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
             acci[j] += (double)si[j];
         }
     }
+}
+
+template <typename QT, int PPT>
+__global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
+    __shared__ int far_pixels;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int bx = blockIdx.x * 16, by = blockIdx.y * (16 * PPT);
+    if (threadIdx.x == 0) far_pixels = 0;
+    __syncthreads();
+
+    double m2x[PPT], m2y[PPT], m2z[PPT], r2[PPT];
+    bool ok[PPT];
+    bool far = false;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const int lx = bx + tx, ly = by + ty + 16 * j;
+        ok[j] = lx < a.width && ly < a.height;
+        const int x = a.x0 + lx, y = a.y0 + ly;
+        // xy_to_pixel = nu * (nv - y - 1) + x (backprojection.cpp:130); r is constant_exterior 0
+        const int64_t px = (int64_t)a.nu * (int64_t)(a.nv - y - 1) + x;
+        const int64_t idx = px - a.r_min;
+        double rx = 0.0, ry = 0.0, rz = 0.0;
+        if (ok[j] && idx >= 0 && idx < a.r_extent) {
+            rx = a.r[idx];
+            ry = a.r[a.r_extent + idx];
+            rz = a.r[2 * a.r_extent + idx];
+        }
+        m2x[j] = -2.0 * rx;
+        m2y[j] = -2.0 * ry;
+        m2z[j] = -2.0 * rz;
+        r2[j] = rx * rx + ry * ry + rz * rz;
+        far = far || !(r2[j] <= a.r_cover2);
+    }
+    if (far) far_pixels = 1;
+    __syncthreads();
+
+    double accr[PPT], acci[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) accr[j] = acci[j] = 0.0;
+    if (far_pixels == 0)
+        accumulate_pulses<QT, PPT, true>(a, m2x, m2y, m2z, r2, accr, acci);
+    else
+        accumulate_pulses<QT, PPT, false>(a, m2x, m2y, m2z, r2, accr, acci);
 
     // fimg = img * expj(k_c * dr_i(pixel, npulses / 2)) (backprojection.cpp:135)
-    const PulseConst cm = a.pc[np / 2];
+    const PulseConst cm = a.pc[a.np / 2];
     const double k_c = a.hdr->k_c;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
@@ -488,14 +536,14 @@ __global__ __launch_bounds__(kSimThreads) void sim_phs_kernel(
 
 // ------------------------------------------------------------------ launchers
 
-hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np, int n_fft,
-                               double delta_r, int norm_policy, PulseConst *pc, BpHeader *hdr,
-                               hipStream_t stream) {
-    const RangeAxis ax = range_axis(ns, n_fft, delta_r);
+hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np,
+                               const RowLayout &lay, int norm_policy, PulseConst *pc,
+                               BpHeader *hdr, hipStream_t stream) {
     const int threads = 256;
     const int blocks = (np + threads - 1) / threads;
     hipLaunchKernelGGL(pulse_const_kernel, dim3(blocks > 0 ? blocks : 1), dim3(threads), 0, stream,
-                       pos, k_r, ns, np, ax.lsa, ax.inv_lsr, norm_policy, pc, hdr);
+                       pos, k_r, ns, np, lay.lsa, lay.lsr, lay.inv_lsr, lay.ext, norm_policy, pc,
+                       hdr);
     return hipGetLastError();
 }
 
@@ -511,36 +559,49 @@ static void allow_max_dynamic_lds(const void *fn) {
 }
 
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
-                                 const double *wp, int ns, int np, int n_fft, const float2 *tw,
-                                 void *q, int q_format, float *qscale, hipStream_t stream) {
+                                 const double *wp, int ns, int np, const RowLayout &lay,
+                                 const float2 *tw, float2 *rot, void *q, int q_format,
+                                 float *qscale, hipStream_t stream) {
+    const int n_fft = lay.n_fft;
     int logN = 0;
     while ((1 << logN) < n_fft) ++logN;
+    hipLaunchKernelGGL(rotation_kernel, dim3((lay.len + 255) / 256), dim3(256), 0, stream, k_r, ns,
+                       lay.lsa, lay.lsr, lay.ext, lay.len, rot);
     const size_t lds = sizeof(float2) * (size_t)n_fft;
     if (q_format == SARBP_Q_C16) {
         static std::once_flag once;
         std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_kernel<QPairF16>); });
         hipLaunchKernelGGL(range_profile_kernel<QPairF16>, dim3(np), dim3(kFftThreads), lds, stream,
-                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (QPairF16 *)q,
-                           qscale);
+                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
+                           (const float2 *)rot, lay.ext, lay.len, (QPairF16 *)q, qscale);
     } else {
         static std::once_flag once;
         std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_kernel<QPairF32>); });
         hipLaunchKernelGGL(range_profile_kernel<QPairF32>, dim3(np), dim3(kFftThreads), lds, stream,
-                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw, (QPairF32 *)q,
-                           (float *)nullptr);
+                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
+                           (const float2 *)rot, lay.ext, lay.len, (QPairF32 *)q, (float *)nullptr);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream) {
-    constexpr int PPT = 4;
+template <int PPT>
+static hipError_t launch_accumulate_ppt(const BpArgs &a, hipStream_t stream) {
     const dim3 grid((a.width + 15) / 16, (a.height + 16 * PPT - 1) / (16 * PPT));
-    if (a.width <= 0 || a.height <= 0) return hipSuccess;
     if (a.q_format == SARBP_Q_C16)
         hipLaunchKernelGGL((accumulate_kernel<QPairF16, PPT>), grid, dim3(kBpThreads), 0, stream, a);
     else
         hipLaunchKernelGGL((accumulate_kernel<QPairF32, PPT>), grid, dim3(kBpThreads), 0, stream, a);
     return hipGetLastError();
+}
+
+hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream) {
+    if (a.width <= 0 || a.height <= 0) return hipSuccess;
+    switch (a.ppt) {
+    case 2: return launch_accumulate_ppt<2>(a, stream);
+    case 3: return launch_accumulate_ppt<3>(a, stream);
+    case 4: return launch_accumulate_ppt<4>(a, stream);
+    default: return launch_accumulate_ppt<kDefaultPpt>(a, stream);
+    }
 }
 
 hipError_t launch_img_output(const double2 *img, int64_t npix, double *dB, uint8_t *u8,

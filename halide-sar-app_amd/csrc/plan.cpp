@@ -16,6 +16,8 @@ struct sarbp_plan {
     double *wp = nullptr;       // Taylor window over pulses, f64 [np]
     void *q = nullptr;          // range profiles [np][N] in q_format
     float *qscale = nullptr;    // per-profile inverse scale of f16 profiles [np]
+    float2 *rot = nullptr;      // row rotation R_j [len]
+    RowLayout lay;              // row layout and interpolation axis
     PulseConst *pc = nullptr;   // per-pulse constants [np]
     BpHeader *hdr = nullptr;    // launch-wide constants
     const float2 *tw = nullptr; // twiddles exp(-2 pi i t / N)
@@ -70,6 +72,10 @@ int check_desc(const sarbp_plan_desc *d) {
         set_error("invalid policy or format in plan descriptor");
         return halide_error_code_constraint_violated;
     }
+    if (d->tile_ppt < 0 || d->tile_ppt > 4 || d->tile_ppt == 1) {
+        set_error("tile_ppt %d is not one of 0 (default), 2, 3, 4", d->tile_ppt);
+        return halide_error_code_constraint_violated;
+    }
     if (d->range_mode != SARBP_RANGE_F64) {
         set_error("range_mode %d is not available; use SARBP_RANGE_F64", d->range_mode);
         return halide_error_code_constraint_violated;
@@ -121,6 +127,7 @@ extern "C" int sarbp_plan_destroy(sarbp_plan *p) {
     (void)hipFree(p->wp);
     (void)hipFree(p->q);
     (void)hipFree(p->qscale);
+    (void)hipFree(p->rot);
     (void)hipFree(p->pc);
     (void)hipFree(p->hdr);
     delete p;
@@ -152,9 +159,12 @@ extern "C" int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out) {
     taylor_window(ns, d->taylor_s_l, d->taylor_den_policy, ws.data());
     taylor_window(np, d->taylor_s_l, d->taylor_den_policy, wp.data());
     const size_t qelem = q_elem_bytes(d->q_format);
+    p->lay = row_layout(ns, N, d->delta_r);
+    const int len = p->lay.len;
     bool ok = hipMalloc(&p->ws, sizeof(double) * ns) == hipSuccess &&
               hipMalloc(&p->wp, sizeof(double) * np) == hipSuccess &&
-              hipMalloc(&p->q, qelem * (size_t)np * (size_t)(N + kQPad)) == hipSuccess &&
+              hipMalloc(&p->q, qelem * (size_t)np * (size_t)len) == hipSuccess &&
+              hipMalloc(&p->rot, sizeof(float2) * (size_t)len) == hipSuccess &&
               hipMalloc(&p->pc, sizeof(PulseConst) * (size_t)np) == hipSuccess &&
               hipMalloc(&p->hdr, sizeof(BpHeader)) == hipSuccess &&
               (d->q_format != SARBP_Q_C16 ||
@@ -181,6 +191,10 @@ extern "C" int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out) {
 }
 
 extern "C" void *sarbp_plan_q_buffer(sarbp_plan *p) { return p ? p->q : nullptr; }
+
+extern "C" size_t sarbp_plan_q_bytes(sarbp_plan *p) {
+    return p ? q_elem_bytes(p->d.q_format) * (size_t)p->d.npulses * (size_t)p->lay.len : 0;
+}
 
 namespace {
 float half_to_float(uint16_t h) {
@@ -217,13 +231,15 @@ extern "C" int sarbp_plan_read_profiles(sarbp_plan *p, float *host_out) {
         return halide_error_code_buffer_argument_is_null;
     }
     DeviceGuard g(p->d.device);
-    const int np = p->d.npulses, N = p->d.n_fft;
+    const int np = p->d.npulses, N = p->d.n_fft, len = p->lay.len, ext = p->lay.ext;
     const size_t eb = q_elem_bytes(p->d.q_format);
-    const size_t row = (size_t)(N + kQPad) * eb;
+    const size_t row = (size_t)len * eb;
     std::vector<uint8_t> h(row * (size_t)np);
     std::vector<float> sc((size_t)np, 1.0f);
+    std::vector<float2> rot((size_t)len);
     if (hipDeviceSynchronize() != hipSuccess ||
         hipMemcpy(h.data(), p->q, h.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(rot.data(), p->rot, sizeof(float2) * len, hipMemcpyDeviceToHost) != hipSuccess ||
         (p->qscale && hipMemcpy(sc.data(), p->qscale, sizeof(float) * np,
                                 hipMemcpyDeviceToHost) != hipSuccess)) {
         set_error("sarbp_plan_read_profiles: copy failed");
@@ -231,20 +247,22 @@ extern "C" int sarbp_plan_read_profiles(sarbp_plan *p, float *host_out) {
     }
     for (int pp = 0; pp < np; ++pp) {
         for (int k = 0; k < N; ++k) {
-            const uint8_t *e = h.data() + (size_t)pp * row + (size_t)(k + 1) * eb;
+            const int j = k + ext;  // element j holds A_j = Q[j - ext] * R_j
+            const uint8_t *e = h.data() + (size_t)pp * row + (size_t)j * eb;
             float re, im;
             if (p->d.q_format == SARBP_Q_C16) {
                 uint16_t a, b;
                 std::memcpy(&a, e, 2);
                 std::memcpy(&b, e + 2, 2);
-                re = half_to_float(a) * sc[pp];
-                im = half_to_float(b) * sc[pp];
+                re = half_to_float(a);
+                im = half_to_float(b);
             } else {
                 std::memcpy(&re, e, 4);
                 std::memcpy(&im, e + 4, 4);
             }
-            host_out[2 * ((size_t)pp * N + k)] = re;
-            host_out[2 * ((size_t)pp * N + k) + 1] = im;
+            const double cr = rot[j].x, ci = rot[j].y;  // multiply by conj(R_j), |R_j| = 1
+            host_out[2 * ((size_t)pp * N + k)] = (float)((re * cr + im * ci) * sc[pp]);
+            host_out[2 * ((size_t)pp * N + k) + 1] = (float)((im * cr - re * ci) * sc[pp]);
         }
     }
     return 0;
@@ -278,7 +296,7 @@ extern "C" int sarbp_plan_range_profiles(sarbp_plan *p, const void *phs_dev, con
     }
     DeviceGuard g(p->d.device);
     hipError_t e = launch_range_profiles(phs_dev, (const float *)k_r_dev, p->ws, p->wp,
-                                         p->d.nsamples, p->d.npulses, p->d.n_fft, p->tw,
+                                         p->d.nsamples, p->d.npulses, p->lay, p->tw, p->rot,
                                          q_dev ? q_dev : p->q, p->d.q_format, p->qscale,
                                          (hipStream_t)stream);
     if (e != hipSuccess) {
@@ -299,8 +317,7 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
     const sarbp_plan_desc &d = p->d;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = launch_pulse_consts((const float *)pos_dev, (const float *)k_r_dev, d.nsamples,
-                                       d.npulses, d.n_fft, d.delta_r, d.norm_r0_policy, p->pc,
-                                       p->hdr, s);
+                                       d.npulses, p->lay, d.norm_r0_policy, p->pc, p->hdr, s);
     if (e == hipSuccess) {
         BpArgs a;
         a.q = q_dev ? q_dev : p->q;
@@ -310,7 +327,11 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
         a.np = d.npulses;
         a.pc = p->pc;
         a.hdr = p->hdr;
+        a.ext = p->lay.ext;
+        a.len = p->lay.len;
+        a.r_cover2 = p->lay.r_cover * p->lay.r_cover;
         a.r = (const double *)r_dev;
+        a.ppt = d.tile_ppt;
         a.r_min = d.r_min;
         a.r_extent = d.r_extent;
         a.nu = d.nu;

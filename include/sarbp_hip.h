@@ -114,7 +114,9 @@ typedef struct sarbp_plan_desc {
     int32_t q_format;                /* sarbp_q_format */
     int32_t range_mode;              /* sarbp_range_mode */
     int32_t device;                  /* HIP device ordinal */
-    int32_t reserved[7];
+    int32_t tile_ppt;                /* pixels per thread of the accumulation tile (2, 3 or 4;
+                                        0 picks the default) */
+    int32_t reserved[6];
 } sarbp_plan_desc;
 
 void sarbp_plan_desc_init(sarbp_plan_desc *d);
@@ -130,14 +132,15 @@ int sarbp_plan_execute(sarbp_plan *p, const void *phs_dev, const void *k_r_dev,
                        const void *pos_dev, const void *r_dev, void *out_dev, void *stream);
 
 /* The two stages separately (for profiling and tests). q_dev may be NULL to use the plan's own
- * range-profile buffer; otherwise it must hold npulses * (n_fft + 4) stored elements of
- * 16 bytes (SARBP_Q_C32) or 8 bytes (SARBP_Q_C16): the padded (q, dq) rows the kernels use. */
+ * range-profile buffer; otherwise it must hold sarbp_plan_q_bytes(p) bytes (the plan's stored
+ * row format: npulses rows of edge-extended, phase-rotated interpolation elements). */
 int sarbp_plan_range_profiles(sarbp_plan *p, const void *phs_dev, const void *k_r_dev,
                               void *q_dev, void *stream);
 int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const void *k_r_dev,
                           const void *pos_dev, const void *r_dev, void *out_dev, void *stream);
-/* Device pointer of the plan's range-profile buffer. */
+/* Device pointer and size of the plan's range-profile buffer. */
 void *sarbp_plan_q_buffer(sarbp_plan *p);
+size_t sarbp_plan_q_bytes(sarbp_plan *p);
 /* Copy the plan's range profiles Q to host memory as complex f32 [npulses][n_fft] (undoing the
  * padded storage and, for f16, the per-profile scale); synchronises the device. */
 int sarbp_plan_read_profiles(sarbp_plan *p, float *host_out);

@@ -63,7 +63,7 @@ def main():
         raise SystemExit("bench.py needs a GPU")
 
     sc = scenes.config_scene(args.config)
-    grid = pipe.scene_grid(sc)
+    grid = pipe.scene_grid(sc, with_pixel_locs=False)
     log(f"bench: {args.config} np={sc.npulses} ns={sc.nsamples} image {grid['nu']}x{grid['nv']}")
     ds = pipe.DeviceScene(sc, grid)
     qf = sar.Q_C16 if args.q_format == "c16" else sar.Q_C32
@@ -130,8 +130,10 @@ def main():
     rng = np.random.default_rng(12345)
     ys = rng.integers(0, grid["nv"], args.check_pixels)
     xs = rng.integers(0, grid["nu"], args.check_pixels)
-    ref = oracle.backproject_points(Q, sc.nsamples, sc.delta_r, kc, sc.pos, nr0,
-                                    grid["pixel_locs"], grid["nu"], grid["nv"], xs, ys,
+    pts = sar.grid_points(grid, xs, ys)  # one-row geometry holding the sampled pixels
+    cnt = len(xs)
+    ref = oracle.backproject_points(Q, sc.nsamples, sc.delta_r, kc, sc.pos, nr0, pts, cnt, 1,
+                                    np.arange(cnt, dtype=np.int32), np.zeros(cnt, np.int32),
                                     nthreads=nthreads)
     got = img[ys, xs]
     rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
@@ -142,7 +144,9 @@ def main():
         rows = min(args.cpu_rows, grid["nv"])
         y0 = grid["nv"] // 2 - rows // 2
         t1 = time.perf_counter()
-        oracle.backproject(Q, sc.nsamples, sc.delta_r, kc, sc.pos, nr0, grid["pixel_locs"],
+        band = sar.image_grid(grid["nu"], grid["nv"], grid["d_u"], grid["d_v"], grid["u_hat"],
+                              grid["v_hat"]) if grid.get("pixel_locs") is None else grid
+        oracle.backproject(Q, sc.nsamples, sc.delta_r, kc, sc.pos, nr0, band["pixel_locs"],
                            grid["nu"], grid["nv"], region=(0, y0, grid["nu"], rows),
                            nthreads=nthreads)
         t_rows = time.perf_counter() - t1

@@ -167,6 +167,7 @@ _lib.sarbp_plan_read_pulse_norms.argtypes = [_P, _P, _i32]
 _lib.sarbp_plan_read_profiles.argtypes = [_P, _P]
 _lib.sarbp_img_output_device.argtypes = [_P, _i64, _P, _P, _f64, _f64, _P]
 _lib.sarbp_sim_phs_device.argtypes = [_P, _i32, _P, _i32, _P, _P, _i32, _P, _P]
+_lib.sarbp_pixel_locs_device.argtypes = [_P, _i32, _P, _i32, _P, _P, _P, _P]
 
 
 def lib():
@@ -284,17 +285,29 @@ def img_plane_create(nsamples: int, npulses: int, delta_r: float, R_c, n_hat=(0.
     return out
 
 
-def image_grid(nu: int, nv: int, d_u: float, d_v: float, u_hat, v_hat) -> dict:
+def image_grid(nu: int, nv: int, d_u: float, d_v: float, u_hat, v_hat,
+               with_pixel_locs: bool = True) -> dict:
     """An explicit nu x nv grid in the img_plane_create conventions (for configs whose image size
-    is not the img_plane rule's)."""
+    is not the img_plane rule's): u = (x + floor(-nu/2)) d_u, likewise v."""
     u = (np.arange(nu) + (-(nu // 2) if nu % 2 == 0 else -(nu // 2) - 1)) * d_u
     v = (np.arange(nv) + (-(nv // 2) if nv % 2 == 0 else -(nv // 2) - 1)) * d_v
     u_hat = np.asarray(u_hat, np.float64)
     v_hat = np.asarray(v_hat, np.float64)
-    pl = np.empty((3, nu * nv))
-    for d in range(3):
-        pl[d] = ((u_hat[d] * u)[None, :] + (v_hat[d] * v)[:, None]).ravel()
+    pl = None
+    if with_pixel_locs:
+        pl = np.empty((3, nu * nv))
+        for d in range(3):
+            pl[d] = ((0.0 + u_hat[d] * u)[None, :] + (v_hat[d] * v)[:, None]).ravel()
     return dict(nu=nu, nv=nv, d_u=d_u, d_v=d_v, u=u, v=v, u_hat=u_hat, v_hat=v_hat, pixel_locs=pl)
+
+
+def grid_points(grid: dict, xs, ys) -> np.ndarray:
+    """pixel_locs of output pixels (xs, ys) as a (3, count) array: output row y holds v index
+    nv - 1 - y (backprojection.cpp:130)."""
+    xs = np.asarray(xs)
+    vy = grid["nv"] - 1 - np.asarray(ys)
+    u, v = grid["u"][xs], grid["v"][vy]
+    return np.stack([(0.0 + grid["u_hat"][d] * u) + grid["v_hat"][d] * v for d in range(3)])
 
 
 # ---------------------------------------------------------------- device-resident plans
@@ -406,3 +419,15 @@ def sim_phs_device(pos64, k_r64, targets, amps, phs, stream=None):
                                    nt, _ptr(phs), _stream_handle(stream))
     if rv != 0:
         raise RuntimeError(f"sarbp_sim_phs_device failed ({rv}): {last_error()}")
+
+
+def pixel_locs_device(u, v, u_hat, v_hat, out, stream=None):
+    """pixel_locs (img_plane.cpp:77-113) on the device: u, v device float64 tensors, u_hat/v_hat
+    3-vectors on the host, out a device float64 tensor (3, len(u) * len(v))."""
+    uh = np.ascontiguousarray(u_hat, np.float64)
+    vh = np.ascontiguousarray(v_hat, np.float64)
+    rv = _lib.sarbp_pixel_locs_device(_ptr(u), int(u.shape[0]), _ptr(v), int(v.shape[0]),
+                                      uh.ctypes.data_as(_P), vh.ctypes.data_as(_P), _ptr(out),
+                                      _stream_handle(stream))
+    if rv != 0:
+        raise RuntimeError(f"sarbp_pixel_locs_device failed ({rv}): {last_error()}")

@@ -82,6 +82,9 @@ hipError_t launch_img_output(const double2 *img, int64_t npix, double *dB, uint8
                              hipStream_t stream);
 hipError_t launch_u8_only(const double *dB, int64_t npix, uint8_t *u8, double lo, double hi,
                           hipStream_t stream);
+hipError_t launch_pixel_locs(const double *u, int nu, const double *v, int nv,
+                             const double *u_hat, const double *v_hat, double *out,
+                             hipStream_t stream);
 hipError_t launch_sim_phs(const double *pos, int np, const double *k64, int ns,
                           const double *tgt, const double *amp, int nt, float2 *phs,
                           hipStream_t stream);

@@ -497,6 +497,36 @@ __global__ __launch_bounds__(kImgThreads) void u8_kernel(const double *__restric
         u8[i] = to_u8(dB[i], lo, hi);
 }
 
+// ------------------------------------------------------------------ image plane on the device
+
+// pixel_locs(x, d) = sum_r A(r, d) b(x, r) = (0 + u_hat[d] u[x % nu]) + v_hat[d] v[x / nu]
+// (img_plane.cpp:77-113), f64, each product and sum rounded once as on the host.
+__global__ void pixel_locs_kernel(const double *__restrict__ u, int nu, const double *__restrict__ v,
+                                  int nv, double uh0, double uh1, double uh2, double vh0,
+                                  double vh1, double vh2, double *__restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t npix = (int64_t)nu * nv;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double uu = u[i % nu], vv = v[i / nu];
+        out[i] = (0.0 + uh0 * uu) + vh0 * vv;
+        out[npix + i] = (0.0 + uh1 * uu) + vh1 * vv;
+        out[2 * npix + i] = (0.0 + uh2 * uu) + vh2 * vv;
+    }
+}
+
+hipError_t launch_pixel_locs(const double *u, int nu, const double *v, int nv,
+                             const double *u_hat, const double *v_hat, double *out,
+                             hipStream_t stream) {
+    const int64_t npix = (int64_t)nu * nv;
+    if (npix <= 0) return hipSuccess;
+    const int64_t want = (npix + 255) / 256;
+    const int blocks = (int)(want < 65536 ? want : 65536);
+    hipLaunchKernelGGL(pixel_locs_kernel, dim3(blocks), dim3(256), 0, stream, u, nu, v, nv,
+                       u_hat[0], u_hat[1], u_hat[2], v_hat[0], v_hat[1], v_hat[2], out);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ scene simulator
 
 constexpr int kSimThreads = 256;

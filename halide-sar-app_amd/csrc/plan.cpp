@@ -411,3 +411,23 @@ extern "C" int sarbp_sim_phs_device(const double *pos_dev, int32_t npulses,
     }
     return 0;
 }
+
+extern "C" int sarbp_pixel_locs_device(const double *u_dev, int32_t nu, const double *v_dev,
+                                       int32_t nv, const double *u_hat, const double *v_hat,
+                                       double *pixel_locs_dev, void *stream) {
+    if (!u_dev || !v_dev || !u_hat || !v_hat || !pixel_locs_dev) {
+        set_error("sarbp_pixel_locs_device: NULL argument");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    if (nu < 0 || nv < 0) {
+        set_error("sarbp_pixel_locs_device: negative extent");
+        return halide_error_code_buffer_extents_negative;
+    }
+    hipError_t e = launch_pixel_locs(u_dev, nu, v_dev, nv, u_hat, v_hat, pixel_locs_dev,
+                                     (hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_error("pixel_locs launch failed: %s", hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    return 0;
+}

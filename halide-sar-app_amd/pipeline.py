@@ -33,7 +33,13 @@ class DeviceScene:
         self.n_fft = _pkg.nfft(sc.nsamples, sc.upsample)
         self.k_r = torch.from_numpy(np.ascontiguousarray(sc.k_r, np.float32)).to(self.dev)
         self.pos = torch.from_numpy(np.ascontiguousarray(sc.pos, np.float32)).to(self.dev)
-        self.r = torch.from_numpy(np.ascontiguousarray(grid["pixel_locs"], np.float64)).to(self.dev)
+        if grid.get("pixel_locs") is not None:
+            self.r = torch.from_numpy(np.ascontiguousarray(grid["pixel_locs"], np.float64)).to(self.dev)
+        else:
+            u = torch.from_numpy(np.ascontiguousarray(grid["u"], np.float64)).to(self.dev)
+            v = torch.from_numpy(np.ascontiguousarray(grid["v"], np.float64)).to(self.dev)
+            self.r = torch.empty((3, self.nu * self.nv), dtype=torch.float64, device=self.dev)
+            _pkg.pixel_locs_device(u, v, grid["u_hat"], grid["v_hat"], self.r)
         if phs_host is not None:
             self.phs = torch.from_numpy(np.ascontiguousarray(phs_host, np.complex64)).to(self.dev)
         elif sc.phs is not None:
@@ -78,13 +84,16 @@ def simulate_device(sc, dev):
     return phs
 
 
-def scene_grid(sc) -> dict:
-    """The scene's image grid: an explicit grid if the scene names one, else img_plane_create."""
-    ip = _pkg.img_plane_create(sc.nsamples, sc.npulses, sc.delta_r, sc.R_c, with_pixel_locs=False)
+def scene_grid(sc, with_pixel_locs: bool = True) -> dict:
+    """The scene's image grid: an explicit grid if the scene names one, else img_plane_create.
+    Without host pixel_locs, DeviceScene generates them on the device."""
+    ip = _pkg.img_plane_create(sc.nsamples, sc.npulses, sc.delta_r, sc.R_c,
+                               with_pixel_locs=with_pixel_locs and getattr(sc, "image", None) is None)
     if getattr(sc, "image", None) is not None:
         nu, nv, du, dv = sc.image
-        return _pkg.image_grid(nu, nv, du, dv, ip["u_hat"], ip["v_hat"])
-    return _pkg.img_plane_create(sc.nsamples, sc.npulses, sc.delta_r, sc.R_c)
+        return _pkg.image_grid(nu, nv, du, dv, ip["u_hat"], ip["v_hat"],
+                               with_pixel_locs=with_pixel_locs)
+    return ip
 
 
 def pixel_pulses(sc, grid) -> float:

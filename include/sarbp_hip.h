@@ -101,6 +101,13 @@ int sarbp_img_plane_create(int32_t nsamples, int32_t npulses, double delta_r, co
                            int32_t *dims_out, double *scal_out, double *u, double *v, double *k_u,
                            double *k_v, double *u_hat, double *v_hat, double *pixel_locs);
 
+/* Image-plane pixel locations generated on the device (img_plane.cpp:77-113):
+ * pixel_locs(x, d) = u_hat[d] * u[x % nu] + v_hat[d] * v[x / nu], written as f64 {nu*nv, 3}
+ * (stride (1, nu*nv)). u_dev/v_dev are device arrays; u_hat/v_hat are host arrays of 3. */
+int sarbp_pixel_locs_device(const double *u_dev, int32_t nu, const double *v_dev, int32_t nv,
+                            const double *u_hat, const double *v_hat, double *pixel_locs_dev,
+                            void *stream);
+
 /* Device-resident plan: everything the hot path needs besides the per-call inputs. */
 typedef struct sarbp_plan sarbp_plan;
 

@@ -269,3 +269,42 @@ def test_pulse_norm_table_bit_exact(sar, orc, scenes, torch_dev, policy):
     bad = np.count_nonzero(got != ref)
     print(f"policy {policy}: {bad} of {n} norms differ")
     assert bad == 0
+
+
+def test_pixels_beyond_row_cover_take_exact_path(sar, orc, scenes, torch_dev):
+    """Pixels farther from the scene centre than the stored rows reach (the reference clamps their
+    interpolation index to the profile ends, backprojection.cpp:59-62, but keeps the exact phase).
+    The image plane is stretched 4x so that part of the image lies beyond the rows' extension while
+    the centre stays inside; both the fast and the exact path run in one image."""
+    sc = scenes.tiny_scene(npulses=24, nsamples=40, seed=77)
+    ip = orc.img_plane(sc.nsamples, sc.npulses, sc.delta_r, sc.R_c)
+    nu, nv = ip["nu"], ip["nv"]
+    pl = ip["pixel_locs"].reshape(3, nv, nu).copy()
+    pl[:, :, nu // 2:] *= 4.0          # right half beyond the rows, left half inside
+    pl = np.ascontiguousarray(pl.reshape(3, nu * nv))
+    n_fft = 128
+    img = run_abi(sar, sc.phs, sc.k_r, 17, n_fft, sc.delta_r, nu, nv, sc.pos, pl)
+    ref = orc.backprojection(sc.phs, sc.k_r, 17, n_fft, sc.delta_r, nu, nv, sc.pos, pl)
+    lsb = np.floor(sc.nsamples * sc.delta_r / 2)
+    radius = np.sqrt((pl ** 2).sum(0)).max()
+    print(f"far pixels: max |r| {radius:.1f} m vs profile half-width {lsb:.1f} m")
+    assert radius > 2.5 * lsb
+    e = rel_l2(img, ref)
+    print(f"far pixels: rel_l2 {e:.3e}")
+    assert e < TOL
+
+
+def test_device_pixel_locs_match_host(sar, torch_dev):
+    """pixel_locs generated on the device (img_plane.cpp:77-113) equal img_plane_create's host
+    values bit for bit (AFRL-sized plane and an odd-sized explicit grid)."""
+    torch = torch_dev
+    f = np.load(os.path.join(GOLDEN, "afrl_image_plane.npz"))
+    ip = sar.img_plane_create(int(f["nsamples"]), int(f["npulses"]), float(f["delta_r"]), f["R_c"])
+    g2 = sar.image_grid(37, 21, 0.11, 0.13, ip["u_hat"], ip["v_hat"])
+    for g in (ip, g2):
+        u = torch.from_numpy(g["u"]).cuda()
+        v = torch.from_numpy(g["v"]).cuda()
+        out = torch.empty((3, g["nu"] * g["nv"]), dtype=torch.float64, device="cuda")
+        sar.pixel_locs_device(u, v, g["u_hat"], g["v_hat"], out)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), g["pixel_locs"])

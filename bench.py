@@ -49,6 +49,12 @@ def main():
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the steps, run one plain device-to-device copy of this many MiB "
                          "(reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--serving", default="none", choices=["none", "C4", "C5"],
+                    help="also run a serving job stream and add its latency/throughput figures")
+    ap.add_argument("--serving-jobs", type=int, default=0,
+                    help="cut the serving stream to its first N jobs (0: the whole stream)")
+    ap.add_argument("--serving-load", type=float, default=0.7,
+                    help="Poisson arrival rate as a fraction of the measured capacity")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -194,7 +200,26 @@ def main():
                      "flop_per_launch": flop},
         "cpu_baseline": cpu,
     }
+    if args.serving != "none":
+        line["serving"] = run_serving(args)
     print(json.dumps(line), flush=True)
+
+
+def run_serving(args) -> dict:
+    """Serving figures: p50/p99 latency per job and jobs/s of the continuous-batching engine and of
+    the one-job-per-launch configuration, both at the same Poisson arrival rate."""
+    sv = importlib.import_module("halide-sar-app_amd.serving")
+    specs = sv.c4_specs() if args.serving == "C4" else sv.c5_specs()
+    if args.serving_jobs:
+        specs = specs[:args.serving_jobs]
+    res = sv.benchmark(specs, load=args.serving_load, log=log)
+
+    def rnd(d):
+        return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
+    return {"config": args.serving, "jobs": len(specs), "load": args.serving_load,
+            "arrival_rate_hz": round(res["arrival_rate_hz"], 2),
+            "engine": {k: rnd(v) for k, v in res["serving"].items()},
+            "one_job_per_launch": {k: rnd(v) for k, v in res["one_job_per_launch"].items()}}
 
 
 if __name__ == "__main__":

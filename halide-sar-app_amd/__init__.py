@@ -57,6 +57,7 @@ HALIDE_INT, HALIDE_UINT, HALIDE_FLOAT = 0, 1, 2
 
 # halide_error_code_t values returned by the entry points
 ERR = dict(success=0, generic_error=-1, bad_type=-3, access_out_of_bounds=-4,
+           buffer_allocation_too_large=-5,
            buffer_extents_too_large=-6, constraint_violated=-8, param_too_small=-9,
            param_too_large=-10, out_of_memory=-11, buffer_argument_is_null=-12,
            copy_to_host_failed=-14, copy_to_device_failed=-15, device_malloc_failed=-16,

@@ -77,6 +77,14 @@ struct BpArgs {
     double2 *out;
 };
 hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream);
+// A tile of a fused launch: job index and the tile's top-left output pixel within that job's region.
+struct FusedTile {
+    int job, bx, by, pad;
+};
+hipError_t launch_accumulate_fused(const BpArgs *jobs_dev, const FusedTile *tiles_dev, int ntiles,
+                                   int q_format, int ppt, hipStream_t stream);
+// Output rows covered by one accumulation tile for a pixels-per-thread setting (16 columns wide).
+int tile_height(int ppt);
 hipError_t launch_img_output(const double2 *img, int64_t npix, double *dB, uint8_t *u8,
                              double lo, double hi, unsigned long long *scratch_max,
                              hipStream_t stream);

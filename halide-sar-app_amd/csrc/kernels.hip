@@ -374,11 +374,10 @@ __device__ inline void accumulate_pulses(const BpArgs &a, const double (&m2x)[PP
     }
 }
 
+// One 16 x (16 * PPT) tile whose top-left output pixel is (bx, by) relative to the region.
 template <typename QT, int PPT>
-__global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
-    __shared__ int far_pixels;
+__device__ inline void accumulate_tile(const BpArgs &a, int bx, int by, int &far_pixels) {
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int bx = blockIdx.x * 16, by = blockIdx.y * (16 * PPT);
     if (threadIdx.x == 0) far_pixels = 0;
     __syncthreads();
 
@@ -432,6 +431,22 @@ __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
         a.out[(size_t)ly * a.width + lx] =
             make_double2(accr[j] * cs - acci[j] * sn, accr[j] * sn + acci[j] * cs);
     }
+}
+
+template <typename QT, int PPT>
+__global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
+    __shared__ int far_pixels;
+    accumulate_tile<QT, PPT>(a, blockIdx.x * 16, blockIdx.y * (16 * PPT), far_pixels);
+}
+
+// Several jobs in one launch: block b computes tile tiles[b] of job tiles[b].job.
+template <typename QT, int PPT>
+__global__ __launch_bounds__(kBpThreads) void accumulate_fused_kernel(
+    const BpArgs *__restrict__ jobs, const FusedTile *__restrict__ tiles) {
+    __shared__ int far_pixels;
+    const FusedTile t = tiles[blockIdx.x];
+    const BpArgs a = jobs[t.job];
+    accumulate_tile<QT, PPT>(a, t.bx, t.by, far_pixels);
 }
 
 // ------------------------------------------------------------------ image output
@@ -623,6 +638,30 @@ static hipError_t launch_accumulate_ppt(const BpArgs &a, hipStream_t stream) {
         hipLaunchKernelGGL((accumulate_kernel<QPairF32, PPT>), grid, dim3(kBpThreads), 0, stream, a);
     return hipGetLastError();
 }
+
+template <int PPT>
+static hipError_t launch_fused_ppt(const BpArgs *jobs, const FusedTile *tiles, int ntiles,
+                                   int q_format, hipStream_t stream) {
+    if (q_format == SARBP_Q_C16)
+        hipLaunchKernelGGL((accumulate_fused_kernel<QPairF16, PPT>), dim3(ntiles), dim3(kBpThreads),
+                           0, stream, jobs, tiles);
+    else
+        hipLaunchKernelGGL((accumulate_fused_kernel<QPairF32, PPT>), dim3(ntiles), dim3(kBpThreads),
+                           0, stream, jobs, tiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate_fused(const BpArgs *jobs_dev, const FusedTile *tiles_dev, int ntiles,
+                                   int q_format, int ppt, hipStream_t stream) {
+    if (ntiles <= 0) return hipSuccess;
+    switch (ppt) {
+    case 3: return launch_fused_ppt<3>(jobs_dev, tiles_dev, ntiles, q_format, stream);
+    case 4: return launch_fused_ppt<4>(jobs_dev, tiles_dev, ntiles, q_format, stream);
+    default: return launch_fused_ppt<2>(jobs_dev, tiles_dev, ntiles, q_format, stream);
+    }
+}
+
+int tile_height(int ppt) { return 16 * (ppt >= 2 && ppt <= 4 ? ppt : kDefaultPpt); }
 
 hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream) {
     if (a.width <= 0 || a.height <= 0) return hipSuccess;

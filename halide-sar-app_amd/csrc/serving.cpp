@@ -1,0 +1,881 @@
+// Serving engine: continuous batching of image-formation jobs on one GPU.
+//
+// Jobs arrive on host threads (sarbp_server_submit) and are queued in arrival order. A scheduler
+// thread admits them while their device memory fits the HBM budget and the in-flight limit allows;
+// a job that does not fit lets smaller jobs behind it pass, until it has waited kHeadPatienceNs,
+// after which nothing passes it. The budget is one device allocation made when the server is
+// created and divided among admitted jobs by the scheduler (first fit, coalescing), so admission
+// never calls the driver's allocator and jobs on different streams never wait on each other's frees.
+//
+// Work flows through streams with one role each, joined by events, so uploads, kernels and
+// downloads of different jobs overlap:
+//   stage    uploads of every job; image plane, pulse table and range profiles of small jobs
+//            (nu * nv * npulses at most the fusion size)
+//   big[i]   image plane, pulse table, range profiles and accumulation of one larger job at a time
+//            (the least loaded of num_streams streams)
+//   fuse     accumulation of small jobs: staged small jobs gather while every launch-table slot is
+//            in flight, and when one is free, all of them (up to the table's capacity) are computed
+//            by one launch over a job-indexed tile list
+//   dl_small, dl_big   output downloads to the caller's buffers
+// A job's arena space returns to the free list when the scheduler sees its download complete.
+//
+// A per-shape-class autotuner (shape class = N_fft, range-profile format, log2 of pulses and of
+// pixels) times the tile candidates on the first job of a class, over a band of its rows, and uses
+// the fastest for later jobs of that class.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "internal.h"
+
+using namespace sarbp;
+
+namespace {
+
+constexpr int64_t kHeadPatienceNs = 50'000'000;     // 50 ms
+constexpr size_t kFuseSlotBytes = (size_t)4 << 20;  // launch table (job args + tiles) per slot
+constexpr int kFuseSlots = 2;                       // fused launches in flight at most
+// Workgroups per accumulation launch of a large job: two waves of 8 tiles per CU on 256 CUs. While
+// one launch of some 10^5 tiles runs, the device starts no kernel from any other queue (measured
+// with a kernel trace), so a large job is computed in row bands of this many tiles and other jobs'
+// kernels and copies are dispatched between bands.
+constexpr int64_t kBandTiles = 4096;
+constexpr int kBandsAhead = 2;
+
+int64_t now_ns(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() -
+                                                                t0)
+        .count();
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int ilog2(int64_t v) {
+    int l = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++l;
+    }
+    return l;
+}
+
+// First-fit allocator over [0, size) with coalescing of free neighbours; offsets are 256-aligned.
+class DeviceArena {
+  public:
+    void reset(size_t size) {
+        free_.clear();
+        if (size) free_[0] = size;
+    }
+    size_t alloc(size_t bytes) {
+        bytes = align256(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, len = it->second;
+            free_.erase(it);
+            if (len > bytes) free_[off + bytes] = len - bytes;
+            return off;
+        }
+        return SIZE_MAX;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = align256(bytes);
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && off + bytes == next->first) {
+            bytes += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += bytes;
+                return;
+            }
+        }
+        free_[off] = bytes;
+    }
+
+  private:
+    std::map<size_t, size_t> free_;  // offset -> length
+};
+
+struct Layout {  // byte offsets of a job's device buffers inside its arena block
+    size_t phs, k_r, pos, u, v, ws, wp, r, q, qscale, rot, pc, hdr, out, total;
+    size_t host_lo, host_hi;  // [k_r, wp end): the block staged through page-locked host memory
+};
+
+// Page-locked host blocks in power-of-two size classes, reused across jobs: copies from them are
+// asynchronous DMA, where a copy from pageable memory would hold the scheduler until it is staged.
+class PinnedPool {
+  public:
+    void *get(size_t bytes, size_t *cls_out) {
+        size_t cls = 4096;
+        while (cls < bytes) cls <<= 1;
+        *cls_out = cls;
+        auto &lst = free_[cls];
+        if (!lst.empty()) {
+            void *p = lst.back();
+            lst.pop_back();
+            return p;
+        }
+        void *p = nullptr;
+        if (hipHostMalloc(&p, cls, hipHostMallocDefault) != hipSuccess) return nullptr;
+        return p;
+    }
+    void put(void *p, size_t cls) {
+        if (p) free_[cls].push_back(p);
+    }
+    ~PinnedPool() {
+        for (auto &kv : free_)
+            for (void *p : kv.second) (void)hipHostFree(p);
+    }
+
+  private:
+    std::map<size_t, std::vector<void *>> free_;
+};
+
+struct Job {
+    uint64_t id = 0;
+    sarbp_job d{};
+    RowLayout lay{};
+    Layout ar{};
+    std::vector<double> ws, wp;  // host windows (computed at submission)
+    void *stage = nullptr;       // page-locked copy of k_r, pos, u, v, ws, wp
+    size_t stage_cls = 0;
+    int64_t t_submit = 0, t_admit = 0, t_end = 0;
+    int64_t pp = 0;  // pixel-pulses of the job
+    int stream = 0;
+    int ppt = 0;
+    bool fused = false;
+    bool small = false;
+    bool prepared = false;       // image plane, pulse table and range profiles enqueued
+    bool done_recorded = false;  // ev_done is recorded (a never-recorded event queries as complete)
+    int status = 1;              // 1 queued, 2 admitted, 0 done, < 0 error
+    size_t off = SIZE_MAX;       // arena block
+    uint8_t *dev = nullptr;
+    hipEvent_t ev_ready = nullptr;  // small: staged for the fused launch; large: uploaded
+    hipEvent_t ev_acc = nullptr;    // large: accumulated
+    hipEvent_t ev_band[2] = {};     // large: completion of the outstanding accumulation bands
+    bool banding = false;           // large: bands still to enqueue
+    int next_row = 0, bands_enqueued = 0, bands_done = 0;
+    hipEvent_t ev_done = nullptr;   // output in host memory
+    hipEvent_t ev_begin = nullptr;  // timing pair with ev_done: device time of the job
+    int64_t host_ns = 0;            // scheduler time spent enqueuing the job's work
+    float device_ms = 0.f;
+    BpArgs args{};
+    std::tuple<int, int, int, int> cls{};
+};
+
+struct FuseSlot {
+    void *host = nullptr;  // page-locked launch table
+    void *dev = nullptr;   // its device copy
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+};
+
+struct Tuning {
+    int best = 0;          // chosen tile_ppt (0: not decided)
+    bool running = false;  // candidate timings in flight
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> probes;
+};
+
+}  // namespace
+
+struct sarbp_server {
+    sarbp_server_config cfg{};
+    std::chrono::steady_clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Job>> queue;
+    std::vector<std::shared_ptr<Job>> inflight;
+    std::map<uint64_t, std::shared_ptr<Job>> jobs;  // all jobs not yet collected by wait
+    std::vector<Job *> fuse_pending;                // small jobs staged, waiting for a fused launch
+    FuseSlot slots[kFuseSlots];
+    PinnedPool pinned;
+    DeviceArena arena;
+    uint8_t *arena_base = nullptr;
+    std::map<std::tuple<int, int, int, int>, Tuning> tuning;
+    std::vector<hipStream_t> streams;  // big[i]
+    hipStream_t stage = nullptr, fuse_stream = nullptr, dl_small = nullptr, dl_big = nullptr;
+    std::vector<int64_t> stream_load;  // outstanding pixel-pulses per big stream
+    uint64_t next_id = 1;
+    int64_t used_bytes = 0;
+    bool stop = false;
+    std::thread worker;
+    const float2 *tw[16] = {};  // twiddle tables per log2 N_fft
+};
+
+namespace {
+
+Layout layout_of(const sarbp_job &j, const RowLayout &lay) {
+    Layout a{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = off;
+        off += align256(bytes);
+        return at;
+    };
+    const size_t np = (size_t)j.npulses, ns = (size_t)j.nsamples;
+    const size_t npix = (size_t)j.nu * (size_t)j.nv;
+    a.phs = take(8 * np * ns);
+    a.k_r = take(4 * ns);
+    a.pos = take(12 * np);
+    a.u = take(8 * (size_t)j.nu);
+    a.v = take(8 * (size_t)j.nv);
+    a.ws = take(8 * ns);
+    a.wp = take(8 * np);
+    a.host_lo = a.k_r;
+    a.host_hi = a.wp + 8 * np;
+    a.r = take(24 * npix);
+    a.q = take(q_elem_bytes(j.q_format) * np * (size_t)lay.len);
+    a.qscale = take(4 * np);
+    a.rot = take(8 * (size_t)lay.len);
+    a.pc = take(sizeof(PulseConst) * np);
+    a.hdr = take(sizeof(BpHeader));
+    a.out = take(16 * npix);
+    a.total = off;
+    return a;
+}
+
+int check_job(const sarbp_job *j) {
+    if (!j || !j->phs || !j->k_r || !j->pos || !j->out) {
+        set_error("sarbp_server_submit: NULL job or buffer");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    if (j->nsamples < 2 || j->npulses < 2 || j->nu < 1 || j->nv < 1) {
+        set_error("sarbp_server_submit: nsamples, npulses >= 2 and nu, nv >= 1 required");
+        return halide_error_code_param_too_small;
+    }
+    if (j->n_fft < 2 || j->n_fft > kMaxFft || (j->n_fft & (j->n_fft - 1))) {
+        set_error("sarbp_server_submit: N_fft (%d) must be a power of two in [2, %d]", j->n_fft,
+                  kMaxFft);
+        return halide_error_code_param_too_large;
+    }
+    if ((int64_t)j->nu * j->nv > INT32_MAX) {
+        set_error("sarbp_server_submit: image too large");
+        return halide_error_code_buffer_extents_too_large;
+    }
+    if (j->norm_r0_policy < 0 || j->norm_r0_policy > 2 || j->taylor_den_policy < 0 ||
+        j->taylor_den_policy > 1 || j->q_format < 0 || j->q_format > 1) {
+        set_error("sarbp_server_submit: invalid policy or format");
+        return halide_error_code_constraint_violated;
+    }
+    return 0;
+}
+
+// Enqueue the uploads of an admitted job on the stage stream.
+int upload_job(sarbp_server *s, Job &j) {
+    hipStream_t st = s->stage;
+    const sarbp_job &d = j.d;
+    const size_t np = (size_t)d.npulses, ns = (size_t)d.nsamples;
+    const size_t npix = (size_t)d.nu * (size_t)d.nv;
+    uint8_t *b = j.dev;
+    j.stage = s->pinned.get(j.ar.host_hi - j.ar.host_lo, &j.stage_cls);
+    if (!j.stage) {
+        set_error("job %llu: page-locked staging allocation failed", (unsigned long long)j.id);
+        return halide_error_code_out_of_memory;
+    }
+    uint8_t *h = (uint8_t *)j.stage - j.ar.host_lo;  // same offsets as the device block
+    double *u = (double *)(h + j.ar.u), *v = (double *)(h + j.ar.v);
+    const int u0 = d.nu % 2 == 0 ? -(d.nu / 2) : -(d.nu / 2) - 1;  // floor(-n / 2) (img_plane.cpp:12-14)
+    const int v0 = d.nv % 2 == 0 ? -(d.nv / 2) : -(d.nv / 2) - 1;
+    for (int x = 0; x < d.nu; ++x) u[x] = (double)(u0 + x) * d.d_u;
+    for (int x = 0; x < d.nv; ++x) v[x] = (double)(v0 + x) * d.d_v;
+    std::memcpy(h + j.ar.k_r, d.k_r, 4 * ns);
+    std::memcpy(h + j.ar.pos, d.pos, 12 * np);
+    std::memcpy(h + j.ar.ws, j.ws.data(), 8 * ns);
+    std::memcpy(h + j.ar.wp, j.wp.data(), 8 * np);
+    hipError_t e = hipMemcpyAsync(b + j.ar.phs, d.phs, 8 * np * ns, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b + j.ar.host_lo, j.stage, j.ar.host_hi - j.ar.host_lo,
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && d.pixel_locs)
+        e = hipMemcpyAsync(b + j.ar.r, d.pixel_locs, 24 * npix, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) {
+        set_error("job %llu: upload failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
+        return halide_error_code_copy_to_device_failed;
+    }
+    return 0;
+}
+
+// Enqueue image plane, pulse table and range profiles of an uploaded job on `st`, and fill its
+// accumulation arguments.
+int prepare_job(sarbp_server *s, Job &j, hipStream_t st) {
+    const sarbp_job &d = j.d;
+    const size_t npix = (size_t)d.nu * (size_t)d.nv;
+    uint8_t *b = j.dev;
+    hipError_t e = hipSuccess;
+    if (!d.pixel_locs) {
+        e = launch_pixel_locs((const double *)(b + j.ar.u), d.nu, (const double *)(b + j.ar.v),
+                              d.nv, d.u_hat, d.v_hat, (double *)(b + j.ar.r), st);
+    }
+    if (e != hipSuccess) {
+        set_error("job %llu: image plane failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    const int n = d.n_fft;
+    const float *kr = (const float *)(b + j.ar.k_r);
+    e = launch_pulse_consts((const float *)(b + j.ar.pos), kr, d.nsamples, d.npulses, j.lay,
+                            d.norm_r0_policy, (PulseConst *)(b + j.ar.pc), (BpHeader *)(b + j.ar.hdr),
+                            st);
+    if (e == hipSuccess)
+        e = launch_range_profiles(b + j.ar.phs, kr, (const double *)(b + j.ar.ws),
+                                  (const double *)(b + j.ar.wp), d.nsamples, d.npulses, j.lay,
+                                  s->tw[ilog2(n)], (float2 *)(b + j.ar.rot), b + j.ar.q, d.q_format,
+                                  d.q_format == SARBP_Q_C16 ? (float *)(b + j.ar.qscale) : nullptr,
+                                  st);
+    if (e != hipSuccess) {
+        set_error("job %llu: range profiles failed: %s", (unsigned long long)j.id,
+                  hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    BpArgs &a = j.args;
+    a.q = b + j.ar.q;
+    a.q_format = d.q_format;
+    a.qscale = d.q_format == SARBP_Q_C16 ? (const float *)(b + j.ar.qscale) : nullptr;
+    a.n_fft = n;
+    a.np = d.npulses;
+    a.pc = (const PulseConst *)(b + j.ar.pc);
+    a.hdr = (const BpHeader *)(b + j.ar.hdr);
+    a.ext = j.lay.ext;
+    a.len = j.lay.len;
+    a.r_cover2 = j.lay.r_cover * j.lay.r_cover;
+    a.r = (const double *)(b + j.ar.r);
+    a.r_min = 0;
+    a.r_extent = (int64_t)npix;
+    a.nu = d.nu;
+    a.nv = d.nv;
+    a.x0 = 0;
+    a.y0 = 0;
+    a.width = d.nu;
+    a.height = d.nv;
+    a.out = (double2 *)(b + j.ar.out);
+    a.ppt = j.ppt;
+    j.prepared = true;
+    return 0;
+}
+
+// Output download after `after` and the completion event, on the job's download stream. On
+// failure nothing is recorded.
+int finish_job(sarbp_server *s, Job &j, hipEvent_t after) {
+    hipStream_t st = j.small ? s->dl_small : s->dl_big;
+    const size_t npix = (size_t)j.d.nu * (size_t)j.d.nv;
+    hipError_t e = hipStreamWaitEvent(st, after, 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(j.d.out, j.dev + j.ar.out, 16 * npix, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && (e = hipEventRecord(j.ev_done, st)) == hipSuccess) j.done_recorded = true;
+    if (e != hipSuccess) {
+        set_error("job %llu: download failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
+        return halide_error_code_copy_to_host_failed;
+    }
+    return 0;
+}
+
+// A job whose enqueued work failed part way: it completes with `code` once the device is idle
+// (nothing enqueued for it can still touch its arena block or staging buffer afterwards).
+void fail_inflight(sarbp_server *s, Job &j, int code) {
+    (void)s;
+    (void)hipDeviceSynchronize();
+    j.done_recorded = true;  // ev_done is not recorded: it queries as complete
+    j.status = code;
+}
+
+// Row bands of a large job's accumulation. The tiles of the bands are those of one launch over
+// the image, so the result is the same bit for bit. At most kBandsAhead bands are enqueued at a
+// time; the scheduler adds the next band when the oldest completes, so work on any stream that
+// shares a hardware queue with this job's stream waits for at most kBandsAhead bands.
+int band_rows(const BpArgs &a) {
+    const int64_t cols = (a.width + 15) / 16;
+    return (int)std::max<int64_t>(1, kBandTiles / cols) * tile_height(a.ppt);
+}
+
+// Enqueue bands while fewer than kBandsAhead are outstanding; after the last band, the download.
+int advance_bands(sarbp_server *s, Job &j) {
+    hipStream_t st = s->streams[j.stream];
+    while (j.bands_done < j.bands_enqueued &&
+           hipEventQuery(j.ev_band[j.bands_done % kBandsAhead]) == hipSuccess)
+        ++j.bands_done;
+    const int rows = band_rows(j.args);
+    hipError_t e = hipSuccess;
+    while (j.next_row < j.args.height && j.bands_enqueued - j.bands_done < kBandsAhead) {
+        BpArgs a = j.args;
+        a.y0 = j.args.y0 + j.next_row;
+        a.height = std::min(rows, j.args.height - j.next_row);
+        a.out = j.args.out + (size_t)j.next_row * (size_t)j.args.width;
+        e = launch_accumulate(a, st);
+        if (e == hipSuccess) e = hipEventRecord(j.ev_band[j.bands_enqueued % kBandsAhead], st);
+        if (e != hipSuccess) {
+            set_error("job %llu: accumulation failed: %s", (unsigned long long)j.id,
+                      hipGetErrorString(e));
+            return halide_error_code_device_run_failed;
+        }
+        j.next_row += a.height;
+        ++j.bands_enqueued;
+    }
+    if (j.next_row >= j.args.height && j.banding) {
+        j.banding = false;
+        if ((e = hipEventRecord(j.ev_acc, st)) != hipSuccess) {
+            set_error("job %llu: event record failed: %s", (unsigned long long)j.id,
+                      hipGetErrorString(e));
+            return halide_error_code_device_run_failed;
+        }
+        return finish_job(s, j, j.ev_acc);
+    }
+    return 0;
+}
+
+// Accumulate a job that runs alone; the first job of an undecided shape class first times every
+// tile candidate over a band of its top rows, which the full pass then rewrites.
+int run_alone(sarbp_server *s, Job &j) {
+    hipStream_t st = s->streams[j.stream];
+    Tuning &tu = s->tuning[j.cls];
+    hipError_t e = hipStreamWaitEvent(st, j.ev_ready, 0);
+    if (e != hipSuccess) {
+        set_error("job %llu: stream wait failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    if (!j.prepared) {
+        int rv = prepare_job(s, j, st);
+        if (rv) return rv;
+    }
+    if (s->cfg.autotune && tu.best == 0 && !tu.running) {
+        tu.running = true;
+        const int64_t row_pp = (int64_t)j.d.nu * j.d.npulses;
+        const int band = (int)std::min<int64_t>(
+            j.d.nv, std::max<int64_t>(192, (4000000000LL / row_pp) / 192 * 192));
+        for (int ppt : {4, 3, 2}) {
+            hipEvent_t a, b;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            BpArgs args = j.args;
+            args.ppt = ppt;
+            args.height = band;
+            (void)hipEventRecord(a, st);
+            e = launch_accumulate(args, st);
+            (void)hipEventRecord(b, st);
+            tu.probes.push_back({ppt, {a, b}});
+            if (e != hipSuccess) break;
+        }
+        j.ppt = 2;
+    }
+    if (e != hipSuccess) {
+        set_error("job %llu: accumulation failed: %s", (unsigned long long)j.id,
+                  hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    j.args.ppt = j.ppt;
+    j.banding = true;
+    j.next_row = 0;
+    for (int k = 0; k < kBandsAhead; ++k)
+        if (!j.ev_band[k] && hipEventCreateWithFlags(&j.ev_band[k], hipEventDisableTiming) != hipSuccess) {
+            set_error("job %llu: event creation failed", (unsigned long long)j.id);
+            return halide_error_code_device_run_failed;
+        }
+    return advance_bands(s, j);
+}
+
+size_t tiles_of(const Job &j, int th) {
+    return (size_t)((j.d.nv + th - 1) / th) * (size_t)((j.d.nu + 15) / 16);
+}
+
+// One accumulation launch over the tiles of `group` (same format and tile shape, table within the
+// slot), using the free launch-table slot `slot`. Every job of the group is finished or failed.
+void run_fused(sarbp_server *s, FuseSlot &slot, std::vector<Job *> &group) {
+    const int ppt = group[0]->ppt;
+    const int th = tile_height(ppt);
+    size_t ntiles = 0;
+    for (Job *j : group) ntiles += tiles_of(*j, th);
+    const size_t ba = align256(sizeof(BpArgs) * group.size());
+    BpArgs *args = (BpArgs *)slot.host;
+    FusedTile *tiles = (FusedTile *)((uint8_t *)slot.host + ba);
+    size_t t = 0;
+    for (size_t k = 0; k < group.size(); ++k) {
+        const Job *j = group[k];
+        args[k] = j->args;
+        for (int by = 0; by < j->d.nv; by += th)
+            for (int bx = 0; bx < j->d.nu; bx += 16) tiles[t++] = {(int)k, bx, by, 0};
+    }
+    hipStream_t fs = s->fuse_stream;
+    hipError_t e = hipSuccess;
+    for (Job *j : group)
+        if (e == hipSuccess) e = hipStreamWaitEvent(fs, j->ev_ready, 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(slot.dev, slot.host, ba + sizeof(FusedTile) * ntiles,
+                           hipMemcpyHostToDevice, fs);
+    if (e == hipSuccess)
+        e = launch_accumulate_fused((const BpArgs *)slot.dev,
+                                    (const FusedTile *)((uint8_t *)slot.dev + ba), (int)ntiles,
+                                    group[0]->d.q_format, ppt, fs);
+    if (e == hipSuccess) e = hipEventRecord(slot.ev, fs);
+    if (e != hipSuccess) {
+        set_error("fused accumulation failed: %s", hipGetErrorString(e));
+        for (Job *j : group) fail_inflight(s, *j, halide_error_code_device_run_failed);
+        return;
+    }
+    slot.busy = true;
+    for (Job *j : group) {
+        j->fused = true;
+        int r = finish_job(s, *j, slot.ev);
+        if (r) fail_inflight(s, *j, r);
+    }
+}
+
+void retire(sarbp_server *s, Job &j) {
+    s->arena.release(j.off, j.ar.total);
+    s->used_bytes -= (int64_t)j.ar.total;
+    if (!j.small) s->stream_load[j.stream] -= j.pp;
+    s->pinned.put(j.stage, j.stage_cls);
+    j.stage = nullptr;
+    (void)hipEventDestroy(j.ev_done);
+    (void)hipEventDestroy(j.ev_ready);
+    (void)hipEventDestroy(j.ev_acc);
+    (void)hipEventDestroy(j.ev_begin);
+    for (hipEvent_t &ev : j.ev_band)
+        if (ev) (void)hipEventDestroy(ev);
+}
+
+void scheduler(sarbp_server *s) {
+    (void)hipSetDevice(s->cfg.device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        // 1. completions
+        bool progressed = false;
+        for (size_t i = 0; i < s->inflight.size();) {
+            Job &j = *s->inflight[i];
+            if (j.done_recorded && hipEventQuery(j.ev_done) == hipSuccess) {
+                j.t_end = now_ns(s->t0);
+                if (j.status > 0) j.status = 0;
+                (void)hipEventElapsedTime(&j.device_ms, j.ev_begin, j.ev_done);
+                retire(s, j);
+                s->inflight.erase(s->inflight.begin() + (long)i);
+                progressed = true;
+            } else {
+                ++i;
+            }
+        }
+        for (FuseSlot &sl : s->slots)
+            if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) sl.busy = false;
+        for (auto &jp : s->inflight) {
+            Job &j = *jp;
+            if (!j.banding) continue;
+            int rv = advance_bands(s, j);
+            if (rv) {
+                j.banding = false;
+                fail_inflight(s, j, rv);
+            }
+        }
+        for (auto &kv : s->tuning) {
+            Tuning &tu = kv.second;
+            if (!tu.running) continue;
+            bool done = true;
+            for (auto &pr : tu.probes) done = done && hipEventQuery(pr.second.second) == hipSuccess;
+            if (!done) continue;
+            float best_ms = 1e30f;
+            for (auto &pr : tu.probes) {
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, pr.second.first, pr.second.second);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    tu.best = pr.first;
+                }
+                (void)hipEventDestroy(pr.second.first);
+                (void)hipEventDestroy(pr.second.second);
+            }
+            tu.probes.clear();
+            tu.running = false;
+        }
+        if (progressed) s->cv_done.notify_all();
+
+        // 2. admission in arrival order; jobs that fit may pass a head that does not, until the
+        // head has waited kHeadPatienceNs
+        const int64_t now = now_ns(s->t0);
+        for (auto it = s->queue.begin(); it != s->queue.end();) {
+            if ((int)s->inflight.size() >= s->cfg.max_inflight) break;
+            Job &j = **it;
+            const size_t off = s->arena.alloc(j.ar.total);
+            if (off == SIZE_MAX) {
+                if (it == s->queue.begin() && now - j.t_submit > kHeadPatienceNs) break;
+                ++it;
+                continue;
+            }
+            auto sp = *it;
+            it = s->queue.erase(it);
+            const int64_t h0 = now_ns(s->t0);
+            j.t_admit = h0;
+            j.status = 2;
+            j.off = off;
+            j.dev = s->arena_base + off;
+            s->used_bytes += (int64_t)j.ar.total;
+            Tuning &tu = s->tuning[j.cls];
+            j.ppt = tu.best ? tu.best : 2;
+            // a small job of an untuned class runs alone once, on a big stream, to be timed
+            const bool fuse = j.small && (tu.best != 0 || !s->cfg.autotune);
+            j.small = fuse;
+            j.stream = (int)(std::min_element(s->stream_load.begin(), s->stream_load.end()) -
+                             s->stream_load.begin());
+            if (!j.small) s->stream_load[j.stream] += j.pp;
+            (void)hipEventCreateWithFlags(&j.ev_ready, hipEventDisableTiming);
+            (void)hipEventCreateWithFlags(&j.ev_acc, hipEventDisableTiming);
+            (void)hipEventCreate(&j.ev_done);
+            (void)hipEventCreate(&j.ev_begin);
+            (void)hipEventRecord(j.ev_begin, s->stage);
+            s->inflight.push_back(sp);
+            int rv = upload_job(s, j);
+            if (rv == 0 && j.small) rv = prepare_job(s, j, s->stage);
+            if (rv == 0 && hipEventRecord(j.ev_ready, s->stage) != hipSuccess)
+                rv = halide_error_code_device_run_failed;
+            if (rv == 0) {
+                if (j.small) s->fuse_pending.push_back(&j);
+                else rv = run_alone(s, j);
+            }
+            if (rv != 0) fail_inflight(s, j, rv);
+            j.host_ns = now_ns(s->t0) - h0;
+        }
+
+        // 3. small jobs: while every launch-table slot is in flight, staged jobs wait and gather;
+        // when a slot is free, the waiting jobs go into one launch per (format, tile shape), as
+        // many as the slot's table holds. An idle GPU thus starts a small job at once and a busy
+        // one receives them in large batches.
+        for (FuseSlot &sl : s->slots) {
+            if (s->fuse_pending.empty()) break;
+            if (sl.busy) continue;
+            const int fmt = s->fuse_pending[0]->d.q_format, ppt = s->fuse_pending[0]->ppt;
+            const int th = tile_height(ppt);
+            std::vector<Job *> group, rest;
+            size_t bytes = 0;
+            for (Job *j : s->fuse_pending) {
+                const size_t need = sizeof(BpArgs) + 256 + sizeof(FusedTile) * tiles_of(*j, th);
+                if (j->d.q_format == fmt && j->ppt == ppt &&
+                    (group.empty() || bytes + need <= kFuseSlotBytes)) {
+                    group.push_back(j);
+                    bytes += need;
+                } else {
+                    rest.push_back(j);
+                }
+            }
+            s->fuse_pending.swap(rest);
+            if (bytes > kFuseSlotBytes) {  // one job larger than a table: accumulate it alone
+                Job &j = *group[0];
+                j.small = false;  // downloads on dl_big; its stream load is not counted
+                s->stream_load[j.stream] += j.pp;
+                int rv = run_alone(s, j);
+                if (rv) fail_inflight(s, j, rv);
+                continue;
+            }
+            run_fused(s, sl, group);
+        }
+        if (s->stop && s->queue.empty() && s->inflight.empty()) break;
+        if (s->inflight.empty() && s->queue.empty()) {
+            s->cv_work.wait(lk, [s] { return s->stop || !s->queue.empty(); });
+        } else {
+            // poll device progress at 20 microsecond granularity, releasing the lock meanwhile
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            lk.lock();
+        }
+    }
+}
+
+void destroy_resources(sarbp_server *s) {
+    for (auto st : s->streams) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    s->streams.clear();
+    for (hipStream_t *st : {&s->stage, &s->fuse_stream, &s->dl_small, &s->dl_big}) {
+        if (!*st) continue;
+        (void)hipStreamSynchronize(*st);
+        (void)hipStreamDestroy(*st);
+        *st = nullptr;
+    }
+    for (FuseSlot &sl : s->slots) {
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.dev) (void)hipFree(sl.dev);
+        if (sl.host) (void)hipHostFree(sl.host);
+        sl = FuseSlot{};
+    }
+    if (s->arena_base) (void)hipFree(s->arena_base);
+    s->arena_base = nullptr;
+}
+
+}  // namespace
+
+extern "C" void sarbp_server_config_init(sarbp_server_config *c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->device = 0;
+    c->max_inflight = 64;
+    c->hbm_budget_bytes = (int64_t)64 << 30;
+    c->fuse_max_pixel_pulses = (int64_t)1 << 31;
+    c->num_streams = 2;  // streams for large jobs, besides the stage, fuse and download streams
+    c->autotune = 1;
+}
+
+extern "C" int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out) {
+    if (!cfg || !out) {
+        set_error("sarbp_server_create: NULL argument");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    *out = nullptr;
+    if (cfg->max_inflight < 1 || cfg->num_streams < 1 || cfg->num_streams > 16 ||
+        cfg->hbm_budget_bytes <= 0) {
+        set_error("sarbp_server_create: max_inflight >= 1, 1 <= num_streams <= 16 and a positive "
+                  "budget are required");
+        return halide_error_code_constraint_violated;
+    }
+    if (sarbp_device_count() <= cfg->device || cfg->device < 0) {
+        set_error("sarbp_server_create: device %d is not visible", cfg->device);
+        return halide_error_code_no_device_interface;
+    }
+    auto s = std::make_unique<sarbp_server>();
+    s->cfg = *cfg;
+    s->stream_load.assign((size_t)cfg->num_streams, 0);
+    s->t0 = std::chrono::steady_clock::now();
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(cfg->device);
+    int rv = 0;
+    if (hipMalloc((void **)&s->arena_base, (size_t)cfg->hbm_budget_bytes) != hipSuccess) {
+        set_error("sarbp_server_create: the HBM budget (%lld bytes) cannot be allocated",
+                  (long long)cfg->hbm_budget_bytes);
+        s->arena_base = nullptr;
+        (void)hipGetLastError();
+        rv = halide_error_code_device_malloc_failed;
+    }
+    s->arena.reset((size_t)cfg->hbm_budget_bytes);
+    for (int i = 0; rv == 0 && i < cfg->num_streams + 4; ++i) {
+        hipStream_t st;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            set_error("sarbp_server_create: stream creation failed");
+            rv = halide_error_code_device_run_failed;
+            break;
+        }
+        if (i < cfg->num_streams) s->streams.push_back(st);
+        else if (i == cfg->num_streams) s->stage = st;
+        else if (i == cfg->num_streams + 1) s->fuse_stream = st;
+        else if (i == cfg->num_streams + 2) s->dl_small = st;
+        else s->dl_big = st;
+    }
+    for (FuseSlot &sl : s->slots) {
+        if (rv) break;
+        if (hipHostMalloc(&sl.host, kFuseSlotBytes, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&sl.dev, kFuseSlotBytes) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+            set_error("sarbp_server_create: launch table allocation failed");
+            rv = halide_error_code_device_malloc_failed;
+        }
+    }
+    for (int lg = 1; rv == 0 && (1 << lg) <= kMaxFft; ++lg) {
+        s->tw[lg] = twiddles(cfg->device, 1 << lg, nullptr);
+        if (!s->tw[lg]) {
+            set_error("sarbp_server_create: twiddle table allocation failed");
+            rv = halide_error_code_device_malloc_failed;
+        }
+    }
+    if (rv) {
+        destroy_resources(s.get());
+        (void)hipSetDevice(prev);
+        return rv;
+    }
+    (void)hipSetDevice(prev);
+    sarbp_server *raw = s.release();
+    raw->worker = std::thread(scheduler, raw);
+    *out = raw;
+    return 0;
+}
+
+extern "C" int sarbp_server_submit(sarbp_server *s, const sarbp_job *job, uint64_t *job_id) {
+    if (!s) {
+        set_error("sarbp_server_submit: NULL server");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    int rv = check_job(job);
+    if (rv) return rv;
+    auto j = std::make_shared<Job>();
+    j->d = *job;
+    j->lay = row_layout(job->nsamples, job->n_fft, job->delta_r);
+    j->ar = layout_of(*job, j->lay);
+    if ((int64_t)j->ar.total > s->cfg.hbm_budget_bytes) {
+        set_error("sarbp_server_submit: job needs %zu bytes, more than the HBM budget (%lld)",
+                  j->ar.total, (long long)s->cfg.hbm_budget_bytes);
+        return halide_error_code_buffer_allocation_too_large;
+    }
+    j->ws.resize((size_t)job->nsamples);
+    j->wp.resize((size_t)job->npulses);
+    taylor_window(job->nsamples, job->taylor_s_l, job->taylor_den_policy, j->ws.data());
+    taylor_window(job->npulses, job->taylor_s_l, job->taylor_den_policy, j->wp.data());
+    const int64_t pp = (int64_t)job->nu * job->nv * job->npulses;
+    j->pp = pp;
+    j->small = s->cfg.fuse_max_pixel_pulses > 0 && pp <= s->cfg.fuse_max_pixel_pulses;
+    j->cls = std::make_tuple(job->n_fft, job->q_format, ilog2(job->npulses),
+                             ilog2((int64_t)job->nu * job->nv));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        j->id = s->next_id++;
+        j->t_submit = now_ns(s->t0);
+        s->jobs[j->id] = j;
+        s->queue.push_back(j);
+        if (job_id) *job_id = j->id;
+    }
+    s->cv_work.notify_one();
+    return 0;
+}
+
+extern "C" int sarbp_server_wait(sarbp_server *s, uint64_t job_id, int32_t timeout_ms,
+                                 sarbp_job_stats *stats) {
+    if (!s) {
+        set_error("sarbp_server_wait: NULL server");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(job_id);
+    if (it == s->jobs.end()) {
+        set_error("sarbp_server_wait: unknown job %llu", (unsigned long long)job_id);
+        return halide_error_code_generic_error;
+    }
+    std::shared_ptr<Job> j = it->second;
+    auto done = [&] { return j->status <= 0; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, done);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done)) return 1;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->t_submit_ns = j->t_submit;
+        stats->t_admit_ns = j->t_admit;
+        stats->t_end_ns = j->t_end;
+        stats->device_bytes = (int64_t)j->ar.total;
+        stats->fused = j->fused ? 1 : 0;
+        stats->tile_ppt = j->ppt;
+        stats->status = j->status;
+        stats->host_enqueue_us = (int32_t)std::min<int64_t>(j->host_ns / 1000, INT32_MAX);
+        stats->device_us = (int32_t)std::min<double>(1000.0 * j->device_ms, (double)INT32_MAX);
+    }
+    s->jobs.erase(job_id);
+    if (j->status < 0) set_error("job %llu failed (%d)", (unsigned long long)job_id, j->status);
+    return j->status < 0 ? j->status : 0;
+}
+
+extern "C" int sarbp_server_destroy(sarbp_server *s) {
+    if (!s) return 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(s->cfg.device);
+    destroy_resources(s);
+    (void)hipSetDevice(prev);
+    delete s;
+    return 0;
+}

@@ -167,6 +167,63 @@ int sarbp_sim_phs_device(const double *pos_dev, int32_t npulses, const double *k
                          int32_t nsamples, const double *tgt_dev, const double *amp_dev,
                          int32_t ntargets, void *phs_dev, void *stream);
 
+/* ------------------------------------------------------------------ serving engine
+ * Continuous batching of image-formation jobs on one GPU: jobs are admitted in arrival order while
+ * their device memory fits the HBM budget (smaller jobs may pass a head that does not fit), run on
+ * several streams so that transfers and kernels of different jobs overlap, and jobs at or below
+ * `fuse_max_pixel_pulses` share one accumulation launch over a job-indexed tile list. A job's result
+ * is the same whether it runs alone or fused (same per-pixel code). Host buffers of a job must stay
+ * valid until sarbp_server_wait returns for it; page-locked buffers let the copies overlap. */
+typedef struct sarbp_server sarbp_server;
+
+typedef struct sarbp_server_config {
+    int32_t device;
+    int32_t max_inflight;           /* admitted jobs at once (>= 1) */
+    int64_t hbm_budget_bytes;       /* device memory the admitted jobs may hold together */
+    int64_t fuse_max_pixel_pulses;  /* jobs with nu*nv*npulses <= this are fused (0: never) */
+    int32_t num_streams;            /* job streams, 1..16 */
+    int32_t autotune;               /* 1: time the tile candidates on a shape class's first job */
+    int32_t reserved[8];
+} sarbp_server_config;
+
+typedef struct sarbp_job {
+    const void *phs;          /* host complex f32 [npulses][nsamples] */
+    const float *k_r;         /* host f32 [nsamples] */
+    const float *pos;         /* host f32 [npulses][3] */
+    int32_t nsamples, npulses, n_fft, taylor_s_l;
+    double delta_r;
+    int32_t nu, nv;           /* image size; output rows flipped as in backprojection.cpp:130 */
+    double d_u, d_v;          /* grid spacing: u = (x + floor(-nu/2)) d_u, v likewise */
+    double u_hat[3], v_hat[3];
+    const double *pixel_locs; /* optional host f64 {nu*nv, 3}; NULL: generated from the grid */
+    int32_t norm_r0_policy, taylor_den_policy, q_format;
+    int32_t reserved0;
+    void *out;                /* host complex f64 {2, nu, nv} */
+    int32_t reserved[8];
+} sarbp_job;
+
+typedef struct sarbp_job_stats {
+    int64_t t_submit_ns, t_admit_ns, t_end_ns; /* since server creation */
+    int64_t device_bytes;                      /* HBM the job held */
+    int32_t fused;                             /* accumulated in a shared launch */
+    int32_t tile_ppt;                          /* tile shape used */
+    int32_t status;                            /* 0 or a negative error code */
+    int32_t host_enqueue_us;                   /* scheduler time spent enqueuing the job */
+    int32_t device_us;                         /* device time from the job's first operation to
+                                                  its output in host memory */
+    int32_t reserved[3];
+} sarbp_job_stats;
+
+void sarbp_server_config_init(sarbp_server_config *cfg);
+int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out);
+/* Queues a job (returns at once); fails with halide_error_code_buffer_allocation_too_large if the
+ * job alone exceeds the HBM budget. */
+int sarbp_server_submit(sarbp_server *s, const sarbp_job *job, uint64_t *job_id);
+/* Waits for a job (timeout_ms < 0: no limit): 0 when done, 1 on timeout, < 0 if the job failed. */
+int sarbp_server_wait(sarbp_server *s, uint64_t job_id, int32_t timeout_ms, sarbp_job_stats *stats);
+/* Finishes every queued job, then releases the server. */
+int sarbp_server_destroy(sarbp_server *s);
+
 #ifdef __cplusplus
 }
 #endif

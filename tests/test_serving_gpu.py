@@ -1,0 +1,164 @@
+"""Serving engine on the GPU: each job's image is the same alone and in the mix, admission keeps
+to the HBM budget, and a job larger than the budget is rejected at submission."""
+import importlib
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _serving():
+    return importlib.import_module("halide-sar-app_amd.serving")
+
+
+# Mixed shapes: ragged sides (100, 72, 1500), odd pulse counts, three FFT sizes; the pixel-pulse
+# threshold below splits them into fused and stand-alone jobs.
+SPECS = [dict(ns=64, npulses=31, side=48, taylor=17),
+         dict(ns=96, npulses=120, side=100, taylor=35),
+         dict(ns=128, npulses=200, side=128, taylor=22),
+         dict(ns=64, npulses=57, side=64, taylor=30),
+         dict(ns=128, npulses=45, side=72, taylor=19),
+         dict(ns=96, npulses=150, side=128, taylor=33),
+         dict(ns=64, npulses=80, side=48, taylor=25),
+         dict(ns=128, npulses=99, side=100, taylor=28),
+         dict(ns=64, npulses=33, side=64, taylor=31),
+         dict(ns=96, npulses=64, side=80, taylor=18),
+         dict(ns=128, npulses=40, side=1500, taylor=21)]  # 4418 tiles: accumulated in two bands
+FUSE_MAX = 400_000
+
+
+@pytest.fixture(scope="module")
+def factory():
+    return _serving().JobFactory(SPECS, seed_base=9100)
+
+
+def _alone(sar, fac, i):
+    """The job through the reference entry point backprojection() (one job per launch)."""
+    sc, g, sp = fac.scenes[i], fac.grids[i], fac.specs[i]
+    grid = sar.image_grid(sp["side"], sp["side"], g["d_u"], g["d_u"], g["u_hat"], g["v_hat"])
+    out = np.empty((sp["side"], sp["side"]), np.complex128)
+    prev = sar.get_default_policies()
+    sar.set_default_policies(sar.NORM_R0_F32_STRICT, sar.TAYLOR_DEN_F32)
+    try:
+        rv = sar.backprojection(fac.phs[i], sc.k_r, sp["taylor"], sar.nfft(sc.nsamples, 2),
+                                sc.delta_r, grid["u"], grid["v"], sc.pos, grid["pixel_locs"], out)
+    finally:
+        sar.set_default_policies(*prev)
+    assert rv == 0, sar.last_error()
+    return out, grid
+
+
+def _mixed(fac, **cfg):
+    sv = _serving()
+    with sv.Server(fuse_max_pixel_pulses=FUSE_MAX, **cfg) as srv:
+        recs, _, kept = sv.replay(srv, fac, None, keep_outputs=range(len(fac.specs)))
+    return recs, kept
+
+
+def test_alone_equals_mix_bit_exact(sar, factory):
+    """Without tuning every job uses the default tile shape, so fused and stand-alone jobs must
+    reproduce backprojection() bit for bit."""
+    print(f"serving mix of {len(SPECS)} jobs, autotune off", flush=True)
+    recs, kept = _mixed(factory, autotune=False)
+    fused = [r["fused"] for r in recs]
+    assert any(fused) and not all(fused)
+    for i in range(len(SPECS)):
+        alone, _ = _alone(sar, factory, i)
+        assert np.array_equal(kept[i], alone), (i, rel_l2(kept[i], alone))
+
+
+def test_alone_equals_mix_tuned(sar, factory, orc):
+    """With tuning a class may use another tile shape; pixels are computed by the same arithmetic
+    except where a tile's far-pixel test differs, so results agree to 1e-6 relative L2, and with the
+    FP64 oracle to 1e-4."""
+    print(f"serving mix of {len(SPECS)} jobs, autotune on", flush=True)
+    recs, kept = _mixed(factory, autotune=True)
+    for i in range(len(SPECS)):
+        alone, grid = _alone(sar, factory, i)
+        assert rel_l2(kept[i], alone) <= 1e-6, i
+    for i in (1, 4):
+        sc, sp = factory.scenes[i], factory.specs[i]
+        _, grid = _alone(sar, factory, i)
+        ref = orc.backprojection(factory.phs[i], sc.k_r, sp["taylor"], sar.nfft(sc.nsamples, 2),
+                                 sc.delta_r, sp["side"], sp["side"], sc.pos, grid["pixel_locs"],
+                                 int_den=False, nthreads=8)
+        assert rel_l2(kept[i], ref) <= 1e-4, i
+
+
+def test_one_job_at_a_time_config(sar, factory):
+    sv = _serving()
+    print("serving one job per launch", flush=True)
+    with sv.Server(**sv.one_job_per_launch_config()) as srv:
+        recs, _, kept = sv.replay(srv, factory, None, keep_outputs=range(4))
+    assert not any(r["fused"] for r in recs)
+    # admitted one at a time: each job is admitted after the previous one ended
+    order = sorted(recs, key=lambda r: r["t_submit_ms"])
+    for a, b in zip(order, order[1:]):
+        assert b["t_submit_ms"] + b["queue_ms"] >= a["t_end_ms"] - 1e-3
+    for i in range(4):
+        assert np.array_equal(kept[i], _alone(sar, factory, i)[0])
+
+
+def test_admission_keeps_to_budget(sar, factory):
+    """A budget that holds the largest job but not two of them: the admitted jobs' device memory
+    never exceeds it, and every job still completes with the right image."""
+    sv = _serving()
+    print("serving under a small HBM budget", flush=True)
+    sizes = []
+    with sv.Server(hbm_budget_gb=64) as probe:
+        for i in range(len(SPECS)):
+            buf = factory.take_out(SPECS[i]["side"])
+            jid = probe.submit(factory.job(i, buf[0], buf[1]).desc())
+            sizes.append(probe.wait(jid).device_bytes)
+            factory.give_out(SPECS[i]["side"], buf)
+    budget = 1.5 * max(sizes)
+    jobs, bufs, ids = [], [], []
+    with sv.Server(hbm_budget_gb=budget / (1 << 30), fuse_max_pixel_pulses=FUSE_MAX) as srv:
+        for i in range(len(SPECS)):
+            bufs.append(factory.take_out(SPECS[i]["side"]))
+            jobs.append(factory.job(i, bufs[-1][0], bufs[-1][1]))
+            ids.append(srv.submit(jobs[-1].desc()))
+        stats = [srv.wait(j) for j in ids]
+    events = sorted([(s.t_admit_ns, s.device_bytes) for s in stats] +
+                    [(s.t_end_ns, -s.device_bytes) for s in stats], key=lambda e: (e[0], e[1]))
+    held = 0
+    for _, d in events:
+        held += d
+        assert held <= budget
+    for i in (0, 2, 5):
+        assert np.array_equal(bufs[i][0], _alone(sar, factory, i)[0]) or \
+            rel_l2(bufs[i][0], _alone(sar, factory, i)[0]) <= 1e-6
+
+
+def test_job_over_budget_is_rejected(sar, factory):
+    sv = _serving()
+    with sv.Server(hbm_budget_gb=1.0 / 1024) as srv:  # 1 MiB
+        buf = factory.take_out(SPECS[2]["side"])
+        with pytest.raises(sv.ServingError) as e:
+            srv.submit(factory.job(2, buf[0], buf[1]).desc())
+        assert e.value.code == sar.ERR["buffer_allocation_too_large"]
+        # the server still serves jobs that fit
+        small = factory.take_out(SPECS[0]["side"])
+        jid = srv.submit(factory.job(0, small[0], small[1]).desc())
+        assert srv.wait(jid).status == 0
+        assert np.array_equal(small[0], _alone(sar, factory, 0)[0])
+
+
+def test_many_small_jobs_batched(sar):
+    """Jobs arriving while fused launches are in flight wait and are batched into later launches;
+    every one of them completes with the stand-alone image."""
+    sv = _serving()
+    rng = np.random.default_rng(31)
+    specs = [dict(ns=int(rng.choice([64, 96])), npulses=int(rng.integers(20, 90)),
+                  side=int(rng.choice([32, 48, 64])), taylor=int(rng.integers(17, 36)))
+             for _ in range(160)]
+    fac = sv.JobFactory(specs, seed_base=9300)
+    print(f"serving {len(specs)} small jobs at once", flush=True)
+    with sv.Server(autotune=False) as srv:
+        recs, _, kept = sv.replay(srv, fac, None, keep_outputs=range(0, len(specs), 7))
+    assert all(r["fused"] for r in recs)
+    for i in kept:
+        assert np.array_equal(kept[i], _alone(sar, fac, i)[0]), i

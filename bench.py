@@ -49,8 +49,8 @@ def main():
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the steps, run one plain device-to-device copy of this many MiB "
                          "(reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--serving", default="none", choices=["none", "C4", "C5"],
-                    help="also run a serving job stream and add its latency/throughput figures")
+    ap.add_argument("--serving", default="C4", choices=["none", "C4", "C5"],
+                    help="serving job stream whose latency/throughput figures the line carries")
     ap.add_argument("--serving-jobs", type=int, default=0,
                     help="cut the serving stream to its first N jobs (0: the whole stream)")
     ap.add_argument("--serving-load", type=float, default=0.7,

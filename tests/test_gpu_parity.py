@@ -199,6 +199,43 @@ def test_c2_full_size_sampled_and_scaling(sar, orc, scenes, pipe, torch_dev):
     assert torch.equal(out2, out * 2)
 
 
+def test_c3_full_size_sampled(sar, orc, scenes, pipe, torch_dev):
+    """Configuration C3 at full size (8192^2 x 8000 pulses x 4096 samples, N_fft 16384: the largest
+    FFT held in LDS), pixel locations generated on the device: oracle on 1536 sampled pixels
+    (the four image corners' neighbourhoods, one full row across the image edge to edge, random
+    ones)."""
+    torch = torch_dev
+    sc = scenes.config_scene("C3")
+    grid = pipe.scene_grid(sc, with_pixel_locs=False)
+    ds = pipe.DeviceScene(sc, grid)
+    assert ds.n_fft == 16384
+    plan = ds.plan()
+    out = ds.output()
+    print("C3: running 8192^2 x 8000", flush=True)
+    ds.run(plan, out)
+    torch.cuda.synchronize()
+    nu, nv = grid["nu"], grid["nv"]
+    rng = np.random.default_rng(11)
+    xs = np.concatenate([[0, 1, nu - 1, nu - 2, 0, nu - 1, 0, nu - 1], np.arange(0, nu, 16),
+                         rng.integers(0, nu, 1016)])
+    ys = np.concatenate([[0, 0, 0, 0, nv - 1, nv - 1, 1, nv - 2], np.full(nu // 16, nv // 3),
+                         rng.integers(0, nv, 1016)])
+    got = out[torch.from_numpy(ys).to(out.device), torch.from_numpy(xs).to(out.device)].cpu().numpy()
+    assert np.all(np.isfinite(got))
+    phs = ds.phs.cpu().numpy()
+    del out, plan, ds
+    Q = orc.range_profiles(phs, sc.k_r, sc.taylor, 16384)
+    _, nr0 = orc.norm_r0(sc.pos)
+    pts = sar.grid_points(grid, xs, ys)   # a one-row geometry holding just the sampled pixels
+    cnt = len(xs)
+    ref = orc.backproject_points(Q, sc.nsamples, sc.delta_r, sc.k_r[sc.nsamples // 2], sc.pos,
+                                 nr0, pts, cnt, 1, np.arange(cnt, dtype=np.int32),
+                                 np.zeros(cnt, np.int32), nthreads=NTHREADS)
+    e = rel_l2(got, ref)
+    print(f"C3 8192^2 x 8000: sampled rel_l2 {e:.3e} over {cnt} pixels")
+    assert e < TOL
+
+
 def test_img_output_parity(sar, orc, torch_dev):
     torch = torch_dev
     g = np.load(os.path.join(GOLDEN, "c1_64.npz"))

@@ -102,6 +102,19 @@ const float2 *twiddles(int device, int n_fft, hipStream_t stream);
 void release_twiddles();
 
 constexpr int kMaxFft = 16384;
+
+// Plan steps for callers that split the accumulation into row bands (the Halide-buffer entry point
+// overlaps transfers with computation this way). plan_prepare enqueues the range profiles and the
+// pulse table; plan_accumulate_rows enqueues rows [row0, row0 + nrows) of the plan's output
+// region into out_rows (dense complex f64, width x nrows), reading r over [r_min, r_min + r_extent).
+// Bands whose row0 is a multiple of plan_tile_rows give the image of one whole-region launch bit
+// for bit.
+int plan_prepare(sarbp_plan *p, const void *phs_dev, const void *k_r_dev, const void *pos_dev,
+                 hipStream_t stream);
+int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int64_t r_extent,
+                         int row0, int nrows, void *out_rows, hipStream_t stream);
+int plan_tile_rows(const sarbp_plan *p);
+const sarbp_plan_desc &plan_desc(const sarbp_plan *p);
 // Bytes per stored range-profile element: (q, dq) in complex f32 or complex f16.
 inline size_t q_elem_bytes(int q_format) { return q_format == SARBP_Q_C16 ? 8 : 16; }
 

@@ -358,6 +358,63 @@ extern "C" int sarbp_plan_execute(sarbp_plan *p, const void *phs_dev, const void
     return sarbp_plan_accumulate(p, nullptr, k_r_dev, pos_dev, r_dev, out_dev, stream);
 }
 
+namespace sarbp {
+
+int plan_prepare(sarbp_plan *p, const void *phs_dev, const void *k_r_dev, const void *pos_dev,
+                 hipStream_t stream) {
+    int rv = sarbp_plan_range_profiles(p, phs_dev, k_r_dev, nullptr, stream);
+    if (rv) return rv;
+    DeviceGuard g(p->d.device);
+    const sarbp_plan_desc &d = p->d;
+    hipError_t e = launch_pulse_consts((const float *)pos_dev, (const float *)k_r_dev, d.nsamples,
+                                       d.npulses, p->lay, d.norm_r0_policy, p->pc, p->hdr, stream);
+    if (e != hipSuccess) {
+        set_error("pulse-table launch failed: %s", hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    return 0;
+}
+
+int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int64_t r_extent,
+                         int row0, int nrows, void *out_rows, hipStream_t stream) {
+    DeviceGuard g(p->d.device);
+    const sarbp_plan_desc &d = p->d;
+    BpArgs a;
+    a.q = p->q;
+    a.q_format = d.q_format;
+    a.qscale = d.q_format == SARBP_Q_C16 ? p->qscale : nullptr;
+    a.n_fft = d.n_fft;
+    a.np = d.npulses;
+    a.pc = p->pc;
+    a.hdr = p->hdr;
+    a.ext = p->lay.ext;
+    a.len = p->lay.len;
+    a.r_cover2 = p->lay.r_cover * p->lay.r_cover;
+    a.r = r_dev;
+    a.ppt = d.tile_ppt;
+    a.r_min = r_min;
+    a.r_extent = r_extent;
+    a.nu = d.nu;
+    a.nv = d.nv;
+    a.x0 = d.x0;
+    a.y0 = d.y0 + row0;
+    a.width = d.width;
+    a.height = nrows;
+    a.out = (double2 *)out_rows;
+    hipError_t e = launch_accumulate(a, stream);
+    if (e != hipSuccess) {
+        set_error("accumulation launch failed: %s", hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    return 0;
+}
+
+int plan_tile_rows(const sarbp_plan *p) { return tile_height(p->d.tile_ppt); }
+
+const sarbp_plan_desc &plan_desc(const sarbp_plan *p) { return p->d; }
+
+}  // namespace sarbp
+
 namespace {
 std::mutex g_scratch_mu;
 std::vector<unsigned long long *> g_scratch;  // per-device 8-byte max cell

@@ -293,7 +293,15 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
 // ------------------------------------------------------------------ accumulation
 
 constexpr int kBpThreads = 256;
-constexpr int kBpPulseBlock = 64;
+// Pulses per f32 partial sum (flushed to the f64 accumulator after each block) and the unroll
+// factor of the pulse loop; compile-time knobs for tuning sweeps.
+#ifndef SARBP_BP_PULSE_BLOCK
+#define SARBP_BP_PULSE_BLOCK 64
+#endif
+#ifndef SARBP_BP_UNROLL
+#define SARBP_BP_UNROLL 1
+#endif
+constexpr int kBpPulseBlock = SARBP_BP_PULSE_BLOCK;
 constexpr int kDefaultPpt = 2;
 
 // Tile of 16 x (16 * PPT) output pixels per block; thread (tx, ty) owns pixels (tx, ty + 16 j).
@@ -324,6 +332,7 @@ __device__ inline void accumulate_pulses(const BpArgs &a, const double (&m2x)[PP
 #pragma unroll
         for (int j = 0; j < PPT; ++j) sr[j] = si[j] = 0.f;
         const int pe = min(np, p0 + kBpPulseBlock);
+#pragma unroll SARBP_BP_UNROLL
         for (int p = p0; p < pe; ++p) {
             const PulseConst c = a.pc[p];
             const __amdgpu_buffer_rsrc_t row = row_rsrc(qbase + (size_t)p * a.len, row_bytes);

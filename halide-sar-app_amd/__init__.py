@@ -162,6 +162,7 @@ _lib.sarbp_plan_destroy.argtypes = [_P]
 _lib.sarbp_plan_execute.argtypes = [_P, _P, _P, _P, _P, _P, _P]
 _lib.sarbp_plan_range_profiles.argtypes = [_P, _P, _P, _P, _P]
 _lib.sarbp_plan_accumulate.argtypes = [_P, _P, _P, _P, _P, _P, _P]
+_lib.sarbp_plan_set_grid.argtypes = [_P, _f64, _f64, _P, _P]
 _lib.sarbp_plan_q_buffer.argtypes = [_P]
 _lib.sarbp_plan_q_buffer.restype = _P
 _lib.sarbp_plan_read_pulse_norms.argtypes = [_P, _P, _i32]
@@ -364,6 +365,19 @@ class Plan:
         if p is not None and p.value:
             _lib.sarbp_plan_destroy(p)
             self._p = None
+
+    def set_grid(self, d_u, d_v, u_hat, v_hat):
+        """Grid mode (sarbp_plan_set_grid): pixel locations generated in the accumulation kernel
+        from the image grid; execute/accumulate then take r=None. u_hat=None switches back."""
+        if u_hat is None:
+            rv = _lib.sarbp_plan_set_grid(self._p, 0.0, 0.0, None, None)
+        else:
+            uh = np.ascontiguousarray(u_hat, np.float64)
+            vh = np.ascontiguousarray(v_hat, np.float64)
+            rv = _lib.sarbp_plan_set_grid(self._p, float(d_u), float(d_v), uh.ctypes.data,
+                                          vh.ctypes.data)
+        if rv != 0:
+            raise RuntimeError(f"sarbp_plan_set_grid failed ({rv}): {last_error()}")
 
     def execute(self, phs, k_r, pos, r, out, stream=None):
         rv = _lib.sarbp_plan_execute(self._p, _ptr(phs), _ptr(k_r), _ptr(pos), _ptr(r), _ptr(out),

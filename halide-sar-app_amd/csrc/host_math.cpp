@@ -53,6 +53,23 @@ RowLayout row_layout(int ns, int n_fft, double delta_r) {
     return a;
 }
 
+PixelGrid make_grid(int nu, int nv, double d_u, double d_v, const double *u_hat,
+                    const double *v_hat, double *step) {
+    PixelGrid g{};
+    g.on = 1;
+    g.u0 = floordiv(-nu, 2);  // img_plane.cpp:12-14
+    g.v0 = floordiv(-nv, 2);
+    g.d_u = d_u;
+    g.d_v = d_v;
+    for (int c = 0; c < 3; ++c) {
+        g.uh[c] = u_hat[c];
+        g.vh[c] = v_hat[c];
+        // output row y + 16 is grid row vy - 16
+        if (step) step[c] = v_hat[c] * (-16.0 * d_v);
+    }
+    return g;
+}
+
 // Taylor window (signal.h:14-51). The arithmetic follows the Halide expression types:
 // A's pow/acosh in f32, n_bar and sigma_p in f64, the F_m products in f64, and the
 // (m+1)^2/(n+1)^2 ratio of F_m_den as Int(32) division (den_policy 0) or in f32 (1).

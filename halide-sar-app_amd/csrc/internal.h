@@ -18,8 +18,24 @@ struct PulseConst {
     double C;           // |pos_p|^2
     double B;           // (nr0 - lsa) * inv_lsr + ext
     double nr0;         // ||pos_p|| under the selected policy, widened
-    double pad0, pad1;
+    double S;           // 2 step.pos_p for the grid's 16-row step (0 without a grid): in grid mode
+                        // |r + k step - pos|^2 = |r - pos|^2 + (|r + k step|^2 - |r|^2) - k S
+    double pad1;
 };
+
+// A pixel grid generated in the accumulation kernel instead of read from pixel_locs
+// (img_plane.cpp:77-113): r(x, vy) = (0 + u_hat (u0 + x) d_u) + v_hat (v0 + vy) d_v, rounded in that
+// order, for image column x and grid row vy = nv - 1 - y of output row y.
+struct PixelGrid {
+    int on;
+    int u0, v0;  // floor(-nu / 2), floor(-nv / 2)
+    double d_u, d_v;
+    double uh[3], vh[3];
+};
+// The grid of an image nu x nv with spacing d_u, d_v and unit vectors u_hat, v_hat; step receives
+// the location change of 16 output rows (the rows one accumulation thread holds apart).
+PixelGrid make_grid(int nu, int nv, double d_u, double d_v, const double *u_hat,
+                    const double *v_hat, double *step);
 
 // Launch-wide constants written next to the pulse table.
 struct BpHeader {
@@ -54,9 +70,10 @@ void set_error(const char *fmt, ...);
 int taylor_window(int num, int s_l, int den_policy, double *out);
 
 // Kernel launchers (kernels.hip). All are asynchronous on `stream`; return hipError_t.
+// step: the grid's 16-row step (3 doubles, host memory) or NULL.
 hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np,
-                               const RowLayout &lay, int norm_policy, PulseConst *pc,
-                               BpHeader *hdr, hipStream_t stream);
+                               const RowLayout &lay, int norm_policy, const double *step,
+                               PulseConst *pc, BpHeader *hdr, hipStream_t stream);
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
                                  const double *wp, int ns, int np, const RowLayout &lay,
                                  const float2 *tw, float2 *rot, void *q, int q_format,
@@ -75,14 +92,16 @@ struct BpArgs {
     int64_t r_min, r_extent;
     int nu, nv, x0, y0, width, height;
     double2 *out;
+    PixelGrid g;  // g.on: locations from the grid, r unused
 };
 hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream);
 // A tile of a fused launch: job index and the tile's top-left output pixel within that job's region.
 struct FusedTile {
     int job, bx, by, pad;
 };
+// All jobs of one fused launch share q_format, ppt and the grid mode (BpArgs::g.on == grid).
 hipError_t launch_accumulate_fused(const BpArgs *jobs_dev, const FusedTile *tiles_dev, int ntiles,
-                                   int q_format, int ppt, hipStream_t stream);
+                                   int q_format, int ppt, bool grid, hipStream_t stream);
 // Output rows covered by one accumulation tile for a pixels-per-thread setting (16 columns wide).
 int tile_height(int ppt);
 hipError_t launch_img_output(const double2 *img, int64_t npix, double *dB, uint8_t *u8,

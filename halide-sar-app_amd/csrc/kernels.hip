@@ -96,14 +96,21 @@ __device__ inline double fast_sqrt(double s) {
     return __builtin_fma(g, r, g);
 }
 
+// One component of a grid pixel location, rounded as img_plane.cpp:77-113 computes it (product,
+// sum with 0, product, sum; no contraction): bit-identical with pixel_locs_kernel and the host.
+__device__ inline double grid_coord(double uh, double u, double vh, double v) {
+#pragma clang fp contract(off)
+    return (0.0 + uh * u) + vh * v;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ per-pulse constants
 
 __global__ void pulse_const_kernel(const float *__restrict__ pos, const float *__restrict__ k_r,
                                    int ns, int np, double lsa, double lsr, double inv_lsr, int ext,
-                                   int policy, PulseConst *__restrict__ pc,
-                                   BpHeader *__restrict__ hdr) {
+                                   int policy, double sx, double sy, double sz,
+                                   PulseConst *__restrict__ pc, BpHeader *__restrict__ hdr) {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
@@ -140,7 +147,8 @@ __global__ void pulse_const_kernel(const float *__restrict__ pos, const float *_
     // t = luts + ext with luts = (nr0 - d - lsa) / lsr (backprojection.cpp:53-54,117,120)
     c.B = (nr0 - lsa) * inv_lsr + (double)ext;
     c.nr0 = nr0;
-    c.pad0 = c.pad1 = 0.0;
+    c.S = 2.0 * (sx * c.px + sy * c.py + sz * c.pz);
+    c.pad1 = 0.0;
     pc[p] = c;
 }
 
@@ -314,11 +322,13 @@ constexpr int kDefaultPpt = 2;
 // This fast path needs every t of the block inside the row; a block whose pixels reach beyond the
 // row's cover radius takes the exact path instead: j clamped to the row, the phase
 // kappa (t - j) reduced in f64 (the clamped edge elements have B = 0, so the weight is moot).
-template <typename QT, int PPT, bool FAST>
+// GRID: only pixel 0's (m2x, m2y, m2z, r2) are used; pixel j > 0 of the thread lies j grid steps
+// (16 output rows) further and its |r - pos|^2 follows from pixel 0's as s2_0 + dr2[j] - j S_p.
+template <typename QT, int PPT, bool FAST, bool GRID>
 __device__ inline void accumulate_pulses(const BpArgs &a, const double (&m2x)[PPT],
                                          const double (&m2y)[PPT], const double (&m2z)[PPT],
-                                         const double (&r2)[PPT], double (&accr)[PPT],
-                                         double (&acci)[PPT]) {
+                                         const double (&r2)[PPT], const double (&dr2)[PPT],
+                                         double (&accr)[PPT], double (&acci)[PPT]) {
     constexpr bool kScaled = sizeof(QT) == sizeof(QPairF16);
     const double ninv = -a.hdr->inv_lsr;
     const double kappa = a.hdr->kappa;
@@ -339,10 +349,18 @@ __device__ inline void accumulate_pulses(const BpArgs &a, const double (&m2x)[PP
             const float qs = kScaled ? a.qscale[p] : 1.0f;
             int ii[PPT];
             float ww[PPT], xx[PPT];
+            const double s20 = __builtin_fma(
+                m2x[0], c.px, __builtin_fma(m2y[0], c.py, __builtin_fma(m2z[0], c.pz, r2[0] + c.C)));
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
-                const double s2 = __builtin_fma(
-                    m2x[j], c.px, __builtin_fma(m2y[j], c.py, __builtin_fma(m2z[j], c.pz, r2[j] + c.C)));
+                double s2;
+                if (j == 0)
+                    s2 = s20;
+                else if (GRID)
+                    s2 = s20 + __builtin_fma(-(double)j, c.S, dr2[j]);
+                else
+                    s2 = __builtin_fma(m2x[j], c.px,
+                                       __builtin_fma(m2y[j], c.py, __builtin_fma(m2z[j], c.pz, r2[j] + c.C)));
                 const double d = fast_sqrt(s2);
                 const double t = __builtin_fma(ninv, d, c.B);
                 if (FAST) {
@@ -384,33 +402,50 @@ __device__ inline void accumulate_pulses(const BpArgs &a, const double (&m2x)[PP
 }
 
 // One 16 x (16 * PPT) tile whose top-left output pixel is (bx, by) relative to the region.
-template <typename QT, int PPT>
-__device__ inline void accumulate_tile(const BpArgs &a, int bx, int by, int &far_pixels) {
+// Location of output pixel (lx, ly) of the region: from the grid, or read from r (pixel index
+// xy_to_pixel = nu * (nv - y - 1) + x, backprojection.cpp:130; r is constant_exterior 0).
+template <bool GRID>
+__device__ inline void pixel_loc(const BpArgs &a, int lx, int ly, bool ok, double &rx, double &ry,
+                                 double &rz) {
+    const int x = a.x0 + lx, y = a.y0 + ly;
+    if (GRID) {
+        const double u = (double)(a.g.u0 + x) * a.g.d_u;
+        const double v = (double)(a.g.v0 + (a.nv - y - 1)) * a.g.d_v;
+        rx = grid_coord(a.g.uh[0], u, a.g.vh[0], v);
+        ry = grid_coord(a.g.uh[1], u, a.g.vh[1], v);
+        rz = grid_coord(a.g.uh[2], u, a.g.vh[2], v);
+        return;
+    }
+    const int64_t px = (int64_t)a.nu * (int64_t)(a.nv - y - 1) + x;
+    const int64_t idx = px - a.r_min;
+    rx = ry = rz = 0.0;
+    if (ok && idx >= 0 && idx < a.r_extent) {
+        rx = a.r[idx];
+        ry = a.r[a.r_extent + idx];
+        rz = a.r[2 * a.r_extent + idx];
+    }
+}
+
+template <typename QT, int PPT, bool GRID>
+__device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, int &far_pixels) {
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     if (threadIdx.x == 0) far_pixels = 0;
     __syncthreads();
 
-    double m2x[PPT], m2y[PPT], m2z[PPT], r2[PPT];
+    double m2x[PPT], m2y[PPT], m2z[PPT], r2[PPT], dr2[PPT];
     bool ok[PPT];
     bool far = false;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         const int lx = bx + tx, ly = by + ty + 16 * j;
         ok[j] = lx < a.width && ly < a.height;
-        const int x = a.x0 + lx, y = a.y0 + ly;
-        // xy_to_pixel = nu * (nv - y - 1) + x (backprojection.cpp:130); r is constant_exterior 0
-        const int64_t px = (int64_t)a.nu * (int64_t)(a.nv - y - 1) + x;
-        const int64_t idx = px - a.r_min;
-        double rx = 0.0, ry = 0.0, rz = 0.0;
-        if (ok[j] && idx >= 0 && idx < a.r_extent) {
-            rx = a.r[idx];
-            ry = a.r[a.r_extent + idx];
-            rz = a.r[2 * a.r_extent + idx];
-        }
+        double rx, ry, rz;
+        pixel_loc<GRID>(a, lx, ly, ok[j], rx, ry, rz);
         m2x[j] = -2.0 * rx;
         m2y[j] = -2.0 * ry;
         m2z[j] = -2.0 * rz;
         r2[j] = rx * rx + ry * ry + rz * rz;
+        dr2[j] = r2[j] - r2[0];
         far = far || !(r2[j] <= a.r_cover2);
     }
     if (far) far_pixels = 1;
@@ -420,9 +455,9 @@ __device__ inline void accumulate_tile(const BpArgs &a, int bx, int by, int &far
 #pragma unroll
     for (int j = 0; j < PPT; ++j) accr[j] = acci[j] = 0.0;
     if (far_pixels == 0)
-        accumulate_pulses<QT, PPT, true>(a, m2x, m2y, m2z, r2, accr, acci);
+        accumulate_pulses<QT, PPT, true, GRID>(a, m2x, m2y, m2z, r2, dr2, accr, acci);
     else
-        accumulate_pulses<QT, PPT, false>(a, m2x, m2y, m2z, r2, accr, acci);
+        accumulate_pulses<QT, PPT, false, GRID>(a, m2x, m2y, m2z, r2, dr2, accr, acci);
 
     // fimg = img * expj(k_c * dr_i(pixel, npulses / 2)) (backprojection.cpp:135)
     const PulseConst cm = a.pc[a.np / 2];
@@ -430,32 +465,42 @@ __device__ inline void accumulate_tile(const BpArgs &a, int bx, int by, int &far
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         if (!ok[j]) continue;
-        const double rx = -0.5 * m2x[j], ry = -0.5 * m2y[j], rz = -0.5 * m2z[j];
+        const int lx = bx + tx, ly = by + ty + 16 * j;
+        double rx, ry, rz;
+        if (GRID) {
+            pixel_loc<true>(a, lx, ly, true, rx, ry, rz);
+        } else {
+            rx = -0.5 * m2x[j];
+            ry = -0.5 * m2y[j];
+            rz = -0.5 * m2z[j];
+        }
         const double dx = rx - cm.px, dy = ry - cm.py, dz = rz - cm.pz;
         const double dm = sqrt(dx * dx + dy * dy + dz * dz);
         const double ang = k_c * (cm.nr0 - dm);
         double sn, cs;
         sincos(ang, &sn, &cs);
-        const int lx = bx + tx, ly = by + ty + 16 * j;
         a.out[(size_t)ly * a.width + lx] =
             make_double2(accr[j] * cs - acci[j] * sn, accr[j] * sn + acci[j] * cs);
     }
 }
 
-template <typename QT, int PPT>
+// Grid mode is a kernel template parameter (not a branch in one kernel): each variant gets its
+// own register allocation, so the grid kernels keep the occupancy their smaller state allows.
+template <typename QT, int PPT, bool GRID>
 __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
     __shared__ int far_pixels;
-    accumulate_tile<QT, PPT>(a, blockIdx.x * 16, blockIdx.y * (16 * PPT), far_pixels);
+    accumulate_tile_impl<QT, PPT, GRID>(a, blockIdx.x * 16, blockIdx.y * (16 * PPT), far_pixels);
 }
 
-// Several jobs in one launch: block b computes tile tiles[b] of job tiles[b].job.
-template <typename QT, int PPT>
+// Several jobs in one launch: block b computes tile tiles[b] of job tiles[b].job; all the jobs of
+// a launch share the grid mode.
+template <typename QT, int PPT, bool GRID>
 __global__ __launch_bounds__(kBpThreads) void accumulate_fused_kernel(
     const BpArgs *__restrict__ jobs, const FusedTile *__restrict__ tiles) {
     __shared__ int far_pixels;
     const FusedTile t = tiles[blockIdx.x];
     const BpArgs a = jobs[t.job];
-    accumulate_tile<QT, PPT>(a, t.bx, t.by, far_pixels);
+    accumulate_tile_impl<QT, PPT, GRID>(a, t.bx, t.by, far_pixels);
 }
 
 // ------------------------------------------------------------------ image output
@@ -591,13 +636,14 @@ __global__ __launch_bounds__(kSimThreads) void sim_phs_kernel(
 // ------------------------------------------------------------------ launchers
 
 hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np,
-                               const RowLayout &lay, int norm_policy, PulseConst *pc,
-                               BpHeader *hdr, hipStream_t stream) {
+                               const RowLayout &lay, int norm_policy, const double *step,
+                               PulseConst *pc, BpHeader *hdr, hipStream_t stream) {
     const int threads = 256;
     const int blocks = (np + threads - 1) / threads;
+    const double sx = step ? step[0] : 0.0, sy = step ? step[1] : 0.0, sz = step ? step[2] : 0.0;
     hipLaunchKernelGGL(pulse_const_kernel, dim3(blocks > 0 ? blocks : 1), dim3(threads), 0, stream,
-                       pos, k_r, ns, np, lay.lsa, lay.lsr, lay.inv_lsr, lay.ext, norm_policy, pc,
-                       hdr);
+                       pos, k_r, ns, np, lay.lsa, lay.lsr, lay.inv_lsr, lay.ext, norm_policy, sx,
+                       sy, sz, pc, hdr);
     return hipGetLastError();
 }
 
@@ -638,47 +684,64 @@ hipError_t launch_range_profiles(const void *phs, const float *k_r, const double
     return hipGetLastError();
 }
 
-template <int PPT>
+template <int PPT, bool GRID>
 static hipError_t launch_accumulate_ppt(const BpArgs &a, hipStream_t stream) {
     const dim3 grid((a.width + 15) / 16, (a.height + 16 * PPT - 1) / (16 * PPT));
     if (a.q_format == SARBP_Q_C16)
-        hipLaunchKernelGGL((accumulate_kernel<QPairF16, PPT>), grid, dim3(kBpThreads), 0, stream, a);
+        hipLaunchKernelGGL((accumulate_kernel<QPairF16, PPT, GRID>), grid, dim3(kBpThreads), 0,
+                           stream, a);
     else
-        hipLaunchKernelGGL((accumulate_kernel<QPairF32, PPT>), grid, dim3(kBpThreads), 0, stream, a);
+        hipLaunchKernelGGL((accumulate_kernel<QPairF32, PPT, GRID>), grid, dim3(kBpThreads), 0,
+                           stream, a);
     return hipGetLastError();
 }
 
-template <int PPT>
+template <int PPT, bool GRID>
 static hipError_t launch_fused_ppt(const BpArgs *jobs, const FusedTile *tiles, int ntiles,
                                    int q_format, hipStream_t stream) {
     if (q_format == SARBP_Q_C16)
-        hipLaunchKernelGGL((accumulate_fused_kernel<QPairF16, PPT>), dim3(ntiles), dim3(kBpThreads),
-                           0, stream, jobs, tiles);
+        hipLaunchKernelGGL((accumulate_fused_kernel<QPairF16, PPT, GRID>), dim3(ntiles),
+                           dim3(kBpThreads), 0, stream, jobs, tiles);
     else
-        hipLaunchKernelGGL((accumulate_fused_kernel<QPairF32, PPT>), dim3(ntiles), dim3(kBpThreads),
-                           0, stream, jobs, tiles);
+        hipLaunchKernelGGL((accumulate_fused_kernel<QPairF32, PPT, GRID>), dim3(ntiles),
+                           dim3(kBpThreads), 0, stream, jobs, tiles);
     return hipGetLastError();
 }
 
-hipError_t launch_accumulate_fused(const BpArgs *jobs_dev, const FusedTile *tiles_dev, int ntiles,
-                                   int q_format, int ppt, hipStream_t stream) {
-    if (ntiles <= 0) return hipSuccess;
+template <bool GRID>
+static hipError_t launch_fused_mode(const BpArgs *jobs_dev, const FusedTile *tiles_dev, int ntiles,
+                                    int q_format, int ppt, hipStream_t stream) {
     switch (ppt) {
-    case 3: return launch_fused_ppt<3>(jobs_dev, tiles_dev, ntiles, q_format, stream);
-    case 4: return launch_fused_ppt<4>(jobs_dev, tiles_dev, ntiles, q_format, stream);
-    default: return launch_fused_ppt<2>(jobs_dev, tiles_dev, ntiles, q_format, stream);
+    case 3: return launch_fused_ppt<3, GRID>(jobs_dev, tiles_dev, ntiles, q_format, stream);
+    case 4: return launch_fused_ppt<4, GRID>(jobs_dev, tiles_dev, ntiles, q_format, stream);
+    default: return launch_fused_ppt<2, GRID>(jobs_dev, tiles_dev, ntiles, q_format, stream);
     }
+}
+
+hipError_t launch_accumulate_fused(const BpArgs *jobs_dev, const FusedTile *tiles_dev, int ntiles,
+                                   int q_format, int ppt, bool grid, hipStream_t stream) {
+    if (ntiles <= 0) return hipSuccess;
+    return grid ? launch_fused_mode<true>(jobs_dev, tiles_dev, ntiles, q_format, ppt, stream)
+                : launch_fused_mode<false>(jobs_dev, tiles_dev, ntiles, q_format, ppt, stream);
 }
 
 int tile_height(int ppt) { return 16 * (ppt >= 2 && ppt <= 4 ? ppt : kDefaultPpt); }
 
 hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream) {
     if (a.width <= 0 || a.height <= 0) return hipSuccess;
+    if (a.g.on) {
+        switch (a.ppt) {
+        case 2: return launch_accumulate_ppt<2, true>(a, stream);
+        case 3: return launch_accumulate_ppt<3, true>(a, stream);
+        case 4: return launch_accumulate_ppt<4, true>(a, stream);
+        default: return launch_accumulate_ppt<kDefaultPpt, true>(a, stream);
+        }
+    }
     switch (a.ppt) {
-    case 2: return launch_accumulate_ppt<2>(a, stream);
-    case 3: return launch_accumulate_ppt<3>(a, stream);
-    case 4: return launch_accumulate_ppt<4>(a, stream);
-    default: return launch_accumulate_ppt<kDefaultPpt>(a, stream);
+    case 2: return launch_accumulate_ppt<2, false>(a, stream);
+    case 3: return launch_accumulate_ppt<3, false>(a, stream);
+    case 4: return launch_accumulate_ppt<4, false>(a, stream);
+    default: return launch_accumulate_ppt<kDefaultPpt, false>(a, stream);
     }
 }
 

@@ -21,6 +21,8 @@ struct sarbp_plan {
     PulseConst *pc = nullptr;   // per-pulse constants [np]
     BpHeader *hdr = nullptr;    // launch-wide constants
     const float2 *tw = nullptr; // twiddles exp(-2 pi i t / N)
+    PixelGrid grid{};           // grid.on: pixel locations generated in the accumulation kernel
+    double step[3] = {0, 0, 0}; // the grid's 16-row step
 };
 
 namespace {
@@ -309,7 +311,8 @@ extern "C" int sarbp_plan_range_profiles(sarbp_plan *p, const void *phs_dev, con
 extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const void *k_r_dev,
                                      const void *pos_dev, const void *r_dev, void *out_dev,
                                      void *stream) {
-    if (!p || !k_r_dev || !pos_dev || !out_dev || (!r_dev && p->d.r_extent > 0)) {
+    if (!p || !k_r_dev || !pos_dev || !out_dev ||
+        (!r_dev && p->d.r_extent > 0 && !p->grid.on)) {
         set_error("sarbp_plan_accumulate: NULL argument");
         return halide_error_code_buffer_argument_is_null;
     }
@@ -317,7 +320,8 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
     const sarbp_plan_desc &d = p->d;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = launch_pulse_consts((const float *)pos_dev, (const float *)k_r_dev, d.nsamples,
-                                       d.npulses, p->lay, d.norm_r0_policy, p->pc, p->hdr, s);
+                                       d.npulses, p->lay, d.norm_r0_policy,
+                                       p->grid.on ? p->step : nullptr, p->pc, p->hdr, s);
     if (e == hipSuccess) {
         BpArgs a;
         a.q = q_dev ? q_dev : p->q;
@@ -341,12 +345,28 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
         a.width = d.width;
         a.height = d.height;
         a.out = (double2 *)out_dev;
+        a.g = p->grid;
         e = launch_accumulate(a, s);
     }
     if (e != hipSuccess) {
         set_error("accumulation launch failed: %s", hipGetErrorString(e));
         return halide_error_code_device_run_failed;
     }
+    return 0;
+}
+
+extern "C" int sarbp_plan_set_grid(sarbp_plan *p, double d_u, double d_v, const double *u_hat,
+                                   const double *v_hat) {
+    if (!p) {
+        set_error("sarbp_plan_set_grid: NULL plan");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    if (!u_hat || !v_hat) {  // back to pixel locations read from r
+        p->grid = PixelGrid{};
+        p->step[0] = p->step[1] = p->step[2] = 0.0;
+        return 0;
+    }
+    p->grid = make_grid(p->d.nu, p->d.nv, d_u, d_v, u_hat, v_hat, p->step);
     return 0;
 }
 
@@ -367,7 +387,8 @@ int plan_prepare(sarbp_plan *p, const void *phs_dev, const void *k_r_dev, const 
     DeviceGuard g(p->d.device);
     const sarbp_plan_desc &d = p->d;
     hipError_t e = launch_pulse_consts((const float *)pos_dev, (const float *)k_r_dev, d.nsamples,
-                                       d.npulses, p->lay, d.norm_r0_policy, p->pc, p->hdr, stream);
+                                       d.npulses, p->lay, d.norm_r0_policy,
+                                       p->grid.on ? p->step : nullptr, p->pc, p->hdr, stream);
     if (e != hipSuccess) {
         set_error("pulse-table launch failed: %s", hipGetErrorString(e));
         return halide_error_code_device_run_failed;
@@ -401,6 +422,7 @@ int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int6
     a.width = d.width;
     a.height = nrows;
     a.out = (double2 *)out_rows;
+    a.g = p->grid;
     hipError_t e = launch_accumulate(a, stream);
     if (e != hipSuccess) {
         set_error("accumulation launch failed: %s", hipGetErrorString(e));

@@ -235,7 +235,7 @@ Layout layout_of(const sarbp_job &j, const RowLayout &lay) {
     a.wp = take(8 * np);
     a.host_lo = a.k_r;
     a.host_hi = a.wp + 8 * np;
-    a.r = take(24 * npix);
+    a.r = take(j.pixel_locs ? 24 * npix : 0);  // no pixel_locs: generated in the kernel
     a.q = take(q_elem_bytes(j.q_format) * np * (size_t)lay.len);
     a.qscale = take(4 * np);
     a.rot = take(8 * (size_t)lay.len);
@@ -314,19 +314,15 @@ int prepare_job(sarbp_server *s, Job &j, hipStream_t st) {
     const size_t npix = (size_t)d.nu * (size_t)d.nv;
     uint8_t *b = j.dev;
     hipError_t e = hipSuccess;
-    if (!d.pixel_locs) {
-        e = launch_pixel_locs((const double *)(b + j.ar.u), d.nu, (const double *)(b + j.ar.v),
-                              d.nv, d.u_hat, d.v_hat, (double *)(b + j.ar.r), st);
-    }
-    if (e != hipSuccess) {
-        set_error("job %llu: image plane failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
-        return halide_error_code_device_run_failed;
-    }
+    // without pixel_locs the accumulation generates the image plane itself (grid mode)
+    double step[3];
+    const PixelGrid grid = d.pixel_locs ? PixelGrid{}
+                                        : make_grid(d.nu, d.nv, d.d_u, d.d_v, d.u_hat, d.v_hat, step);
     const int n = d.n_fft;
     const float *kr = (const float *)(b + j.ar.k_r);
     e = launch_pulse_consts((const float *)(b + j.ar.pos), kr, d.nsamples, d.npulses, j.lay,
-                            d.norm_r0_policy, (PulseConst *)(b + j.ar.pc), (BpHeader *)(b + j.ar.hdr),
-                            st);
+                            d.norm_r0_policy, grid.on ? step : nullptr, (PulseConst *)(b + j.ar.pc),
+                            (BpHeader *)(b + j.ar.hdr), st);
     if (e == hipSuccess)
         e = launch_range_profiles(b + j.ar.phs, kr, (const double *)(b + j.ar.ws),
                                   (const double *)(b + j.ar.wp), d.nsamples, d.npulses, j.lay,
@@ -360,6 +356,7 @@ int prepare_job(sarbp_server *s, Job &j, hipStream_t st) {
     a.height = d.nv;
     a.out = (double2 *)(b + j.ar.out);
     a.ppt = j.ppt;
+    a.g = grid;
     j.prepared = true;
     return 0;
 }
@@ -514,7 +511,7 @@ void run_fused(sarbp_server *s, FuseSlot &slot, std::vector<Job *> &group) {
     if (e == hipSuccess)
         e = launch_accumulate_fused((const BpArgs *)slot.dev,
                                     (const FusedTile *)((uint8_t *)slot.dev + ba), (int)ntiles,
-                                    group[0]->d.q_format, ppt, fs);
+                                    group[0]->d.q_format, ppt, group[0]->args.g.on != 0, fs);
     if (e == hipSuccess) e = hipEventRecord(slot.ev, fs);
     if (e != hipSuccess) {
         set_error("fused accumulation failed: %s", hipGetErrorString(e));
@@ -642,19 +639,20 @@ void scheduler(sarbp_server *s) {
         }
 
         // 3. small jobs: while every launch-table slot is in flight, staged jobs wait and gather;
-        // when a slot is free, the waiting jobs go into one launch per (format, tile shape), as
-        // many as the slot's table holds. An idle GPU thus starts a small job at once and a busy
-        // one receives them in large batches.
+        // when a slot is free, the waiting jobs go into one launch per (format, tile shape, grid
+        // mode), as many as the slot's table holds. An idle GPU thus starts a small job at once and
+        // a busy one receives them in large batches.
         for (FuseSlot &sl : s->slots) {
             if (s->fuse_pending.empty()) break;
             if (sl.busy) continue;
             const int fmt = s->fuse_pending[0]->d.q_format, ppt = s->fuse_pending[0]->ppt;
+            const int grid = s->fuse_pending[0]->args.g.on;
             const int th = tile_height(ppt);
             std::vector<Job *> group, rest;
             size_t bytes = 0;
             for (Job *j : s->fuse_pending) {
                 const size_t need = sizeof(BpArgs) + 256 + sizeof(FusedTile) * tiles_of(*j, th);
-                if (j->d.q_format == fmt && j->ppt == ppt &&
+                if (j->d.q_format == fmt && j->ppt == ppt && j->args.g.on == grid &&
                     (group.empty() || bytes + need <= kFuseSlotBytes)) {
                     group.push_back(j);
                     bytes += need;

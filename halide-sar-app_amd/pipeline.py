@@ -21,10 +21,14 @@ def _torch():
 
 
 class DeviceScene:
-    """Phase history, k_r, platform positions and pixel locations resident on the GPU."""
+    """Phase history, k_r, platform positions and pixel locations resident on the GPU.
+
+    A grid without host pixel_locs runs in grid mode by default: the accumulation generates the
+    pixel locations itself and r stays None (grid_mode=False keeps an explicit r, generated on the
+    device from the grid)."""
 
     def __init__(self, sc, grid: dict, device: int = 0, phs_host: np.ndarray | None = None,
-                 simulate: bool = True):
+                 simulate: bool = True, grid_mode: bool | None = None):
         torch = _torch()
         self.sc = sc
         self.grid = grid
@@ -33,8 +37,11 @@ class DeviceScene:
         self.n_fft = _pkg.nfft(sc.nsamples, sc.upsample)
         self.k_r = torch.from_numpy(np.ascontiguousarray(sc.k_r, np.float32)).to(self.dev)
         self.pos = torch.from_numpy(np.ascontiguousarray(sc.pos, np.float32)).to(self.dev)
+        self.grid_mode = grid.get("pixel_locs") is None if grid_mode is None else grid_mode
         if grid.get("pixel_locs") is not None:
             self.r = torch.from_numpy(np.ascontiguousarray(grid["pixel_locs"], np.float64)).to(self.dev)
+        elif self.grid_mode:
+            self.r = None
         else:
             u = torch.from_numpy(np.ascontiguousarray(grid["u"], np.float64)).to(self.dev)
             v = torch.from_numpy(np.ascontiguousarray(grid["v"], np.float64)).to(self.dev)
@@ -52,8 +59,12 @@ class DeviceScene:
 
     def plan(self, region=None, **kw) -> "_pkg.Plan":
         sc = self.sc
-        return _pkg.Plan(sc.nsamples, sc.npulses, self.n_fft, sc.taylor, sc.delta_r, self.nu,
-                         self.nv, region=region, **kw)
+        p = _pkg.Plan(sc.nsamples, sc.npulses, self.n_fft, sc.taylor, sc.delta_r, self.nu,
+                      self.nv, region=region, **kw)
+        if self.grid_mode:
+            g = self.grid
+            p.set_grid(g["d_u"], g["d_v"], g["u_hat"], g["v_hat"])
+        return p
 
     def output(self, region=None):
         torch = _torch()

@@ -148,6 +148,16 @@ int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const void *k_r_dev,
 /* Device pointer and size of the plan's range-profile buffer. */
 void *sarbp_plan_q_buffer(sarbp_plan *p);
 size_t sarbp_plan_q_bytes(sarbp_plan *p);
+
+/* Grid mode: the plan's pixel locations come from the image grid (img_plane.cpp:77-113:
+ * r(x, y) = (0 + u_hat u[x]) + v_hat v[y], u[x] = (x + floor(-nu/2)) d_u, v likewise, nu and nv from
+ * the plan) and are generated inside the accumulation kernel, which then needs no r buffer (r_dev
+ * may be NULL in sarbp_plan_execute / sarbp_plan_accumulate). Within a thread's column the kernel
+ * derives the squared range of pixels 16, 32, 48 rows further from the first one's, so the image
+ * agrees with the r-buffer path to rounding (~1e-13 relative), not bit for bit. u_hat or v_hat NULL
+ * switches back to r. */
+int sarbp_plan_set_grid(sarbp_plan *p, double d_u, double d_v, const double *u_hat,
+                        const double *v_hat);
 /* Copy the plan's range profiles Q to host memory as complex f32 [npulses][n_fft] (undoing the
  * padded storage and, for f16, the per-profile scale); synchronises the device. */
 int sarbp_plan_read_profiles(sarbp_plan *p, float *host_out);

@@ -331,6 +331,43 @@ def test_pixels_beyond_row_cover_take_exact_path(sar, orc, scenes, torch_dev):
     assert e < TOL
 
 
+@pytest.mark.parametrize("ppt", [2, 3, 4])
+def test_grid_mode_matches_pixel_locs(sar, orc, scenes, pipe, torch_dev, ppt):
+    """Grid mode (pixel locations generated in the accumulation kernel, squared ranges of a
+    thread's later pixels derived from its first) against the r-buffer path on the same inputs:
+    C1's full 512^2 plane and a ragged 75 x 61 region of it; and against the oracle.
+
+    The two paths round |r - pos|^2 differently in the last bits, and the f64 square root (hardware
+    reciprocal square root + one Newton step, relative error up to ~1e-13, i.e. ~1e-9 m at 13 km)
+    turns that into up to ~4e-7 rad of phase jitter per pixel-pulse: the images agree to 1e-8 over
+    the plane and 1e-7 over the small region, so 1e-6 bounds rounding while any error in the
+    derivation (a wrong step, sign or pixel) costs whole radians."""
+    torch = torch_dev
+    sc = scenes.config_scene("C1")
+    grid = pipe.scene_grid(sc)
+    phs = scenes.simulate_phs_numpy(sc)
+    ds_r = pipe.DeviceScene(sc, grid, phs_host=phs)
+    ds_g = pipe.DeviceScene(sc, dict(grid, pixel_locs=None), phs_host=phs)
+    assert ds_r.r is not None and ds_g.r is None and ds_g.grid_mode
+    for region in (None, (131, 57, 75, 61)):
+        p_r = ds_r.plan(region=region, tile_ppt=ppt)
+        p_g = ds_g.plan(region=region, tile_ppt=ppt)
+        o_r, o_g = ds_r.output(region), ds_g.output(region)
+        ds_r.run(p_r, o_r)
+        ds_g.run(p_g, o_g)
+        torch.cuda.synchronize()
+        a, b = o_r.cpu().numpy(), o_g.cpu().numpy()
+        e = rel_l2(b, a)
+        print(f"grid vs pixel_locs ppt={ppt} region={region}: rel {e:.2e}")
+        assert e < 1e-6
+    ref = orc.backprojection(phs, sc.k_r, sc.taylor, ds_g.n_fft, sc.delta_r, grid["nu"], grid["nv"],
+                             sc.pos, grid["pixel_locs"], nthreads=NTHREADS)
+    full = ds_g.output()
+    ds_g.run(ds_g.plan(tile_ppt=ppt), full)
+    torch.cuda.synchronize()
+    assert rel_l2(full.cpu().numpy(), ref) < TOL
+
+
 def test_device_pixel_locs_match_host(sar, torch_dev):
     """pixel_locs generated on the device (img_plane.cpp:77-113) equal img_plane_create's host
     values bit for bit (AFRL-sized plane and an odd-sized explicit grid)."""

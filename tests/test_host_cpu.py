@@ -18,7 +18,7 @@ REF_DATA = "/root/reference/data"
 
 
 def _platform():
-    return importlib.import_module("halide-sar-app_amd.platform")
+    return importlib.import_module("halide-sar-app_amd.platform_data")
 
 
 def _write(d, **arrays):

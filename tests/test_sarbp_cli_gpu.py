@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _platform():
-    return importlib.import_module("halide-sar-app_amd.platform")
+    return importlib.import_module("halide-sar-app_amd.platform_data")
 
 
 def _expected_u8(orc, pd, taylor, upsample, res, aspect, lo, hi, n_hat=(0.0, 0.0, 1.0)):

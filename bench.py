@@ -209,11 +209,16 @@ def main():
 
     flop = pp * pipe.algorithmic_flops_per_pp()
     achieved = flop / (acc_ms * 1e-3) / 1e12
+    ppt_used = args.tile_ppt if args.tile_ppt in (2, 3, 4) else 2
+    kernel = (f"accumulate_kernel<{'QPairF16' if qf == sar.Q_C16 else 'QPairF32'}, {ppt_used}, "
+              f"{'grid' if ds.grid_mode else 'pixel_locs'}>")
     traffic = None
     pmc = os.path.join(ROOT, "profiles", f"accumulate_{args.config}_hbm_traffic.json")
     if os.path.exists(pmc):
         with open(pmc) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            rec = json.load(f)
+        if rec.get("kernel") == kernel:   # counters of this kernel on this workload only
+            traffic = rec.get("bytes_per_launch")
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -239,7 +244,7 @@ def main():
                      "img_output_dB_u8": round(img_ms, 4)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                     "traffic": traffic, "kernel": "accumulate_kernel",
+                     "traffic": traffic, "kernel": kernel,
                      "flop_per_launch": flop},
         "cpu_baseline": cpu,
         "abi_call": abi,

@@ -409,6 +409,10 @@ __device__ inline void pixel_loc(const BpArgs &a, int lx, int ly, bool ok, doubl
                                  double &rz) {
     const int x = a.x0 + lx, y = a.y0 + ly;
     if (GRID) {
+        // grid regions lie inside the image (sarbp_plan_set_grid checks), so a thread's pixels are
+        // whole grid steps apart; lanes past the region's edge take location 0, as with r
+        rx = ry = rz = 0.0;
+        if (!ok) return;
         const double u = (double)(a.g.u0 + x) * a.g.d_u;
         const double v = (double)(a.g.v0 + (a.nv - y - 1)) * a.g.d_v;
         rx = grid_coord(a.g.uh[0], u, a.g.vh[0], v);

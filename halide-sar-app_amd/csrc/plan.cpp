@@ -366,7 +366,12 @@ extern "C" int sarbp_plan_set_grid(sarbp_plan *p, double d_u, double d_v, const 
         p->step[0] = p->step[1] = p->step[2] = 0.0;
         return 0;
     }
-    p->grid = make_grid(p->d.nu, p->d.nv, d_u, d_v, u_hat, v_hat, p->step);
+    const sarbp_plan_desc &d = p->d;
+    if (d.x0 < 0 || d.y0 < 0 || (int64_t)d.x0 + d.width > d.nu || (int64_t)d.y0 + d.height > d.nv) {
+        set_error("sarbp_plan_set_grid: the plan's output region must lie inside the nu x nv image");
+        return halide_error_code_constraint_violated;
+    }
+    p->grid = make_grid(d.nu, d.nv, d_u, d_v, u_hat, v_hat, p->step);
     return 0;
 }
 

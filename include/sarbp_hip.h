@@ -154,8 +154,9 @@ size_t sarbp_plan_q_bytes(sarbp_plan *p);
  * the plan) and are generated inside the accumulation kernel, which then needs no r buffer (r_dev
  * may be NULL in sarbp_plan_execute / sarbp_plan_accumulate). Within a thread's column the kernel
  * derives the squared range of pixels 16, 32, 48 rows further from the first one's, so the image
- * agrees with the r-buffer path to rounding (~1e-13 relative), not bit for bit. u_hat or v_hat NULL
- * switches back to r. */
+ * agrees with the r-buffer path to rounding (~1e-8 relative), not bit for bit. The plan's output
+ * region must lie inside the image (constraint_violated otherwise). u_hat or v_hat NULL switches
+ * back to r. */
 int sarbp_plan_set_grid(sarbp_plan *p, double d_u, double d_v, const double *u_hat,
                         const double *v_hat);
 /* Copy the plan's range profiles Q to host memory as complex f32 [npulses][n_fft] (undoing the

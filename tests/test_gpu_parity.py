@@ -335,7 +335,8 @@ def test_pixels_beyond_row_cover_take_exact_path(sar, orc, scenes, torch_dev):
 def test_grid_mode_matches_pixel_locs(sar, orc, scenes, pipe, torch_dev, ppt):
     """Grid mode (pixel locations generated in the accumulation kernel, squared ranges of a
     thread's later pixels derived from its first) against the r-buffer path on the same inputs:
-    C1's full 512^2 plane and a ragged 75 x 61 region of it; and against the oracle.
+    C1's full 512^2 plane, a ragged 75 x 61 region of it and the right-hand 64 columns (a region
+    reaching past the image is refused in grid mode); and against the oracle.
 
     The two paths round |r - pos|^2 differently in the last bits, and the f64 square root (hardware
     reciprocal square root + one Newton step, relative error up to ~1e-13, i.e. ~1e-9 m at 13 km)
@@ -349,7 +350,9 @@ def test_grid_mode_matches_pixel_locs(sar, orc, scenes, pipe, torch_dev, ppt):
     ds_r = pipe.DeviceScene(sc, grid, phs_host=phs)
     ds_g = pipe.DeviceScene(sc, dict(grid, pixel_locs=None), phs_host=phs)
     assert ds_r.r is not None and ds_g.r is None and ds_g.grid_mode
-    for region in (None, (131, 57, 75, 61)):
+    with pytest.raises(RuntimeError, match="inside"):
+        ds_g.plan(region=(480, 500, 64, 40))   # reaches past the image: grid mode refuses it
+    for region in (None, (131, 57, 75, 61), (448, 0, 64, 512)):
         p_r = ds_r.plan(region=region, tile_ppt=ppt)
         p_g = ds_g.plan(region=region, tile_ppt=ppt)
         o_r, o_g = ds_r.output(region), ds_g.output(region)

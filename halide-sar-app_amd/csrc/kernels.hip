@@ -4,7 +4,10 @@
 //   range_profile_kernel    window * |k_r|, centred pad, fftshift, LDS-resident radix-4/2 Stockham
 //                           FFT, fftshift -> range profiles Q (backprojection.cpp:80-104, dft.cpp)
 //   accumulate_kernel       per-pixel range, linear interpolation of Q, phase-corrected sum over
-//                           pulses, mid-pulse phase and row flip (backprojection.cpp:52-65,108-138)
+//                           pulses, mid-pulse phase and row flip (backprojection.cpp:52-65,108-138);
+//                           pixel locations read from pixel_locs or generated from the image grid
+//   accumulate_fused_kernel the same over a job-indexed tile list (several images in one launch)
+//   pixel_locs_kernel       the image plane's pixel locations (img_plane.cpp:77-113)
 //   absmax / db_u8 kernels  img_output_to_dB and img_output_u8 (img_output.cpp:10-52)
 //   sim_phs_kernel          seeded point-scatterer phase histories for benchmarks and tests
 #include <hip/hip_fp16.h>

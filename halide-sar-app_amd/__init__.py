@@ -152,7 +152,7 @@ class PlanDesc(ctypes.Structure):
                 ("width", _i32), ("height", _i32), ("r_min", _i64), ("r_extent", _i64),
                 ("norm_r0_policy", _i32), ("taylor_den_policy", _i32), ("q_format", _i32),
                 ("range_mode", _i32), ("device", _i32), ("tile_ppt", _i32),
-                ("reserved", _i32 * 6)]
+                ("fft_blocks", _i32), ("reserved", _i32 * 5)]
 
 
 _lib.sarbp_plan_desc_init.argtypes = [ctypes.POINTER(PlanDesc)]
@@ -165,6 +165,8 @@ _lib.sarbp_plan_accumulate.argtypes = [_P, _P, _P, _P, _P, _P, _P]
 _lib.sarbp_plan_set_grid.argtypes = [_P, _f64, _f64, _P, _P]
 _lib.sarbp_plan_q_buffer.argtypes = [_P]
 _lib.sarbp_plan_q_buffer.restype = _P
+_lib.sarbp_plan_q_bytes.argtypes = [_P]
+_lib.sarbp_plan_q_bytes.restype = ctypes.c_size_t
 _lib.sarbp_plan_read_pulse_norms.argtypes = [_P, _P, _i32]
 _lib.sarbp_plan_read_profiles.argtypes = [_P, _P]
 _lib.sarbp_img_output_device.argtypes = [_P, _i64, _P, _P, _f64, _f64, _P]
@@ -338,7 +340,7 @@ class Plan:
 
     def __init__(self, nsamples, npulses, n_fft, taylor_s_l, delta_r, nu, nv, region=None,
                  r_min=0, r_extent=None, norm_r0_policy=None, taylor_den_policy=None,
-                 q_format=Q_C32, range_mode=RANGE_F64, device=0, tile_ppt=0):
+                 q_format=Q_C32, range_mode=RANGE_F64, device=0, tile_ppt=0, fft_blocks=0):
         d = PlanDesc()
         _lib.sarbp_plan_desc_init(ctypes.byref(d))
         d.nsamples, d.npulses, d.n_fft, d.taylor_s_l = nsamples, npulses, n_fft, taylor_s_l
@@ -354,6 +356,7 @@ class Plan:
             d.taylor_den_policy = taylor_den_policy
         d.q_format, d.range_mode, d.device = q_format, range_mode, device
         d.tile_ppt = tile_ppt
+        d.fft_blocks = fft_blocks
         self.desc = d
         self._p = _P()
         rv = _lib.sarbp_plan_create(ctypes.byref(d), ctypes.byref(self._p))
@@ -417,6 +420,11 @@ class Plan:
     @property
     def q_ptr(self) -> int:
         return int(_lib.sarbp_plan_q_buffer(self._p))
+
+    @property
+    def q_bytes(self) -> int:
+        """Size of the stored range-profile rows (the buffer range_profiles(q=...) fills)."""
+        return int(_lib.sarbp_plan_q_bytes(self._p))
 
 
 def img_output_device(img, npix: int, dB=None, u8=None, dB_min=0.0, dB_max=1.0, stream=None):

@@ -77,7 +77,10 @@ hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int n
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
                                  const double *wp, int ns, int np, const RowLayout &lay,
                                  const float2 *tw, float2 *rot, void *q, int q_format,
-                                 float *qscale, hipStream_t stream);
+                                 float *qscale, hipStream_t stream, int shared_blocks = 0);
+// shared_blocks > 0 (and N_fft <= kMaxFftShared): the profiles are computed by at most that many
+// 256-thread blocks, which suits a device busy with accumulation launches of other streams.
+constexpr int kMaxFftShared = 8192;
 struct BpArgs {
     const void *q;
     int q_format;

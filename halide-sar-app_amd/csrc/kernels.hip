@@ -3,6 +3,8 @@
 //   pulse_const_kernel      ||pos_p|| table + per-pulse constants (backprojection.cpp:108,120-123)
 //   range_profile_kernel    window * |k_r|, centred pad, fftshift, LDS-resident radix-4/2 Stockham
 //                           FFT, fftshift -> range profiles Q (backprojection.cpp:80-104, dft.cpp)
+//   range_profile_shared_kernel  the same with 256-thread blocks looping over pulses, for a device
+//                           shared with accumulation launches (serving engine)
 //   accumulate_kernel       per-pixel range, linear interpolation of Q, phase-corrected sum over
 //                           pulses, mid-pulse phase and row flip (backprojection.cpp:52-65,108-138);
 //                           pixel locations read from pixel_locs or generated from the image grid
@@ -25,7 +27,10 @@ namespace sarbp {
 
 namespace {
 
+// Complex product, each product and sum rounded once (no contraction), so that both range-profile
+// kernels give the same bits whatever the compiler would fuse in either.
 __device__ inline float2 cmul(float2 a, float2 b) {
+#pragma clang fp contract(off)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
@@ -233,25 +238,25 @@ __device__ inline void fft_pass(float2 *buf, int N, int Ns, const float2 *__rest
     __syncthreads();
 }
 
-// One block per range profile p. Source pulse s = (p + 2 ceil(np/2)) mod np: the pulse axis is
-// rolled by fftshift before and after the FFT (backprojection.cpp:98,104).
-template <typename QT>
-__global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
-    const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
-    const double *__restrict__ wp, int ns, int np, int N, int logN,
-    const float2 *__restrict__ tw, const float2 *__restrict__ rot, int ext, int len,
-    QT *__restrict__ q, float *__restrict__ qscale) {
-    extern __shared__ float2 buf[];
-    __shared__ float red[kFftThreads / 64];
-    const int p = blockIdx.x;
-    const int tid = threadIdx.x;
+// Source pulse of profile p: s = (p + 2 ceil(np/2)) mod np; the pulse axis is rolled by fftshift
+// before and after the FFT (backprojection.cpp:98,104).
+__device__ inline int profile_source(int p, int np) {
     const int pmid = np / 2 + np % 2;
-    const int s = (int)(((long long)p + 2LL * pmid) % np);
+    return (int)(((long long)p + 2LL * pmid) % np);
+}
+
+// Fills buf[0, N) with the padded, windowed, fftshifted samples of pulse s.
+template <int T>
+__device__ inline void load_profile_row(float2 *buf, const float2 *__restrict__ phs,
+                                        const float *__restrict__ k_r,
+                                        const double *__restrict__ ws,
+                                        const double *__restrict__ wp, int ns, int N, int s,
+                                        int tid) {
     const int half = N / 2;                 // ceil(N/2) for even N
     const int xpad = floordiv2(N - ns);     // pad's leading offset (signal_complex.h:14)
     const double wps = wp[s];
     const float2 *row = phs + (size_t)s * ns;
-    for (int x = tid; x < N; x += kFftThreads) {
+    for (int x = tid; x < N; x += T) {
         const int xs = (x + half) & (N - 1);  // fftshift (signal_complex.h:24-29)
         const int i = xs - xpad;
         float2 v = make_float2(0.f, 0.f);
@@ -265,6 +270,59 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
         }
         buf[x] = v;
     }
+}
+
+// Writes the transformed row buf[0, N) as profile row p: Q(k) = dft((k + N/2) mod N)
+// (backprojection.cpp:104) as rotated (A, B) elements; with qscale (f16 storage) the row is first
+// scaled by a power of two. `red` holds T / 64 floats. Ends with a barrier.
+template <typename QT, int T>
+__device__ inline void store_profile_row(const float2 *buf, float *red, int N,
+                                         const float2 *__restrict__ rot, int ext, int len,
+                                         QT *__restrict__ q, float *__restrict__ qscale, int p,
+                                         int tid) {
+#pragma clang fp contract(off)
+    const int half = N / 2;
+    QT *dst = q + (size_t)p * len;
+    float scale = 1.0f;
+    if (qscale) {
+        // f16 storage: scale the profile by 2^e so that its largest component is about 2^14
+        // (exact to undo; 11-bit significands keep ~5e-4 relative rounding per value)
+        float m = 0.0f;
+        for (int k = tid; k < N; k += T) m = fmaxf(m, fmaxf(fabsf(buf[k].x), fabsf(buf[k].y)));
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        if ((tid & 63) == 0) red[tid >> 6] = m;
+        __syncthreads();
+        m = red[0];
+        for (int w = 1; w < T / 64; ++w) m = fmaxf(m, red[w]);
+        int e = 0;
+        if (m > 0.0f && m <= 3.0e38f) (void)frexpf(m, &e);
+        e = max(-100, min(100, 14 - e));
+        scale = ldexpf(1.0f, e);
+        if (tid == 0) qscale[p] = ldexpf(1.0f, -e);
+    }
+    for (int j = tid; j < len; j += T) {
+        const int k0 = min(max(j - ext, 0), N - 1), k1 = min(max(j - ext + 1, 0), N - 1);
+        const float2 a = buf[(k0 + half) & (N - 1)], b = buf[(k1 + half) & (N - 1)];
+        const float2 q0 = make_float2(a.x * scale, a.y * scale);
+        const float2 dq = make_float2(b.x * scale - q0.x, b.y * scale - q0.y);
+        const float2 r = rot[j];
+        storeq(dst + j, cmul(q0, r), cmul(dq, r));
+    }
+    __syncthreads();
+}
+
+// One block of 512 threads per range profile p, the FFT in place in one LDS row.
+template <typename QT>
+__global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
+    const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
+    const double *__restrict__ wp, int ns, int np, int N, int logN,
+    const float2 *__restrict__ tw, const float2 *__restrict__ rot, int ext, int len,
+    QT *__restrict__ q, float *__restrict__ qscale) {
+    extern __shared__ float2 buf[];
+    __shared__ float red[kFftThreads / 64];
+    const int p = blockIdx.x;
+    const int tid = threadIdx.x;
+    load_profile_row<kFftThreads>(buf, phs, k_r, ws, wp, ns, N, profile_source(p, np), tid);
     __syncthreads();
     int Ns = 1;
     if (logN & 1) {
@@ -272,32 +330,67 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
         Ns = 2;
     }
     for (; Ns < N; Ns *= 4) fft_pass<4, kMaxFft / 4 / kFftThreads>(buf, N, Ns, tw, tid);
-    QT *dst = q + (size_t)p * len;
-    float scale = 1.0f;
-    if (qscale) {
-        // f16 storage: scale the profile by 2^e so that its largest component is about 2^14
-        // (exact to undo; 11-bit significands keep ~5e-4 relative rounding per value)
-        float m = 0.0f;
-        for (int k = tid; k < N; k += kFftThreads) m = fmaxf(m, fmaxf(fabsf(buf[k].x), fabsf(buf[k].y)));
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-        if ((tid & 63) == 0) red[tid >> 6] = m;
-        __syncthreads();
-        m = red[0];
-        for (int w = 1; w < kFftThreads / 64; ++w) m = fmaxf(m, red[w]);
-        int e = 0;
-        if (m > 0.0f && m <= 3.0e38f) (void)frexpf(m, &e);
-        e = max(-100, min(100, 14 - e));
-        scale = ldexpf(1.0f, e);
-        if (tid == 0) qscale[p] = ldexpf(1.0f, -e);
+    store_profile_row<QT, kFftThreads>(buf, red, N, rot, ext, len, q, qscale, p, tid);
+}
+
+// Variant for a device shared with long-running accumulation workgroups: blocks of 256 threads and
+// few registers, the shape of an accumulation workgroup, so that each one that retires leaves room
+// for one of these; each block transforms the profiles p = blockIdx.x + k gridDim.x. The FFT runs
+// out of place between two LDS rows (2 N float2, N <= kMaxFftShared), one barrier per pass; the
+// arithmetic of every butterfly is that of fft_pass.
+constexpr int kFftSharedThreads = 256;
+
+template <int R>
+__device__ inline void fft_pass_oop(const float2 *src, float2 *dst, int N, int Ns,
+                                    const float2 *__restrict__ tw, int tid) {
+    const int nbf = N / R;
+    const int stride = N / (Ns * R);
+    for (int j = tid; j < nbf; j += kFftSharedThreads) {
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = src[j + r * nbf];
+        const int k = j & (Ns - 1);
+        if (k != 0) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r * k * stride]);
+        }
+        dft_r<R>(v);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
     }
-    // Q(k) = dft((k + N/2) mod N) (backprojection.cpp:104), written as rotated (A, B) elements
-    for (int j = tid; j < len; j += kFftThreads) {
-        const int k0 = min(max(j - ext, 0), N - 1), k1 = min(max(j - ext + 1, 0), N - 1);
-        const float2 a = buf[(k0 + half) & (N - 1)], b = buf[(k1 + half) & (N - 1)];
-        const float2 q0 = make_float2(a.x * scale, a.y * scale);
-        const float2 dq = make_float2(b.x * scale - q0.x, b.y * scale - q0.y);
-        const float2 r = rot[j];
-        storeq(dst + j, cmul(q0, r), cmul(dq, r));
+    __syncthreads();
+}
+
+template <typename QT>
+__global__ __launch_bounds__(kFftSharedThreads) void range_profile_shared_kernel(
+    const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
+    const double *__restrict__ wp, int ns, int np, int N, int logN,
+    const float2 *__restrict__ tw, const float2 *__restrict__ rot, int ext, int len,
+    QT *__restrict__ q, float *__restrict__ qscale) {
+    extern __shared__ float2 buf[];
+    __shared__ float red[kFftSharedThreads / 64];
+    const int tid = threadIdx.x;
+    for (int p = blockIdx.x; p < np; p += gridDim.x) {
+        float2 *a = buf, *b = buf + N;
+        load_profile_row<kFftSharedThreads>(a, phs, k_r, ws, wp, ns, N, profile_source(p, np),
+                                            tid);
+        __syncthreads();
+        int Ns = 1;
+        if (logN & 1) {
+            fft_pass_oop<2>(a, b, N, 1, tw, tid);
+            Ns = 2;
+            float2 *t = a;
+            a = b;
+            b = t;
+        }
+        for (; Ns < N; Ns *= 4) {
+            fft_pass_oop<4>(a, b, N, Ns, tw, tid);
+            float2 *t = a;
+            a = b;
+            b = t;
+        }
+        store_profile_row<QT, kFftSharedThreads>(a, red, N, rot, ext, len, q, qscale, p, tid);
     }
 }
 
@@ -668,12 +761,31 @@ static void allow_max_dynamic_lds(const void *fn) {
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
                                  const double *wp, int ns, int np, const RowLayout &lay,
                                  const float2 *tw, float2 *rot, void *q, int q_format,
-                                 float *qscale, hipStream_t stream) {
+                                 float *qscale, hipStream_t stream, int shared_blocks) {
     const int n_fft = lay.n_fft;
     int logN = 0;
     while ((1 << logN) < n_fft) ++logN;
     hipLaunchKernelGGL(rotation_kernel, dim3((lay.len + 255) / 256), dim3(256), 0, stream, k_r, ns,
                        lay.lsa, lay.lsr, lay.ext, lay.len, rot);
+    if (shared_blocks > 0 && n_fft <= kMaxFftShared) {
+        const size_t lds = 2 * sizeof(float2) * (size_t)n_fft;
+        const dim3 grid((unsigned)std::min(np, shared_blocks));
+        if (q_format == SARBP_Q_C16) {
+            static std::once_flag once;
+            std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_shared_kernel<QPairF16>); });
+            hipLaunchKernelGGL(range_profile_shared_kernel<QPairF16>, grid, dim3(kFftSharedThreads),
+                               lds, stream, (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN,
+                               tw, (const float2 *)rot, lay.ext, lay.len, (QPairF16 *)q, qscale);
+        } else {
+            static std::once_flag once;
+            std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_shared_kernel<QPairF32>); });
+            hipLaunchKernelGGL(range_profile_shared_kernel<QPairF32>, grid, dim3(kFftSharedThreads),
+                               lds, stream, (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN,
+                               tw, (const float2 *)rot, lay.ext, lay.len, (QPairF32 *)q,
+                               (float *)nullptr);
+        }
+        return hipGetLastError();
+    }
     const size_t lds = sizeof(float2) * (size_t)n_fft;
     if (q_format == SARBP_Q_C16) {
         static std::once_flag once;

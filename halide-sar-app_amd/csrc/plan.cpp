@@ -78,6 +78,10 @@ int check_desc(const sarbp_plan_desc *d) {
         set_error("tile_ppt %d is not one of 0 (default), 2, 3, 4", d->tile_ppt);
         return halide_error_code_constraint_violated;
     }
+    if (d->fft_blocks < 0) {
+        set_error("fft_blocks %d is negative", d->fft_blocks);
+        return halide_error_code_constraint_violated;
+    }
     if (d->range_mode != SARBP_RANGE_F64) {
         set_error("range_mode %d is not available; use SARBP_RANGE_F64", d->range_mode);
         return halide_error_code_constraint_violated;
@@ -300,7 +304,7 @@ extern "C" int sarbp_plan_range_profiles(sarbp_plan *p, const void *phs_dev, con
     hipError_t e = launch_range_profiles(phs_dev, (const float *)k_r_dev, p->ws, p->wp,
                                          p->d.nsamples, p->d.npulses, p->lay, p->tw, p->rot,
                                          q_dev ? q_dev : p->q, p->d.q_format, p->qscale,
-                                         (hipStream_t)stream);
+                                         (hipStream_t)stream, p->d.fft_blocks);
     if (e != hipSuccess) {
         set_error("range-profile launch failed: %s", hipGetErrorString(e));
         return halide_error_code_device_run_failed;

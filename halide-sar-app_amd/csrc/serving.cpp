@@ -7,17 +7,22 @@
 // created and divided among admitted jobs by the scheduler (first fit, coalescing), so admission
 // never calls the driver's allocator and jobs on different streams never wait on each other's frees.
 //
-// Work flows through streams with one role each, joined by events, so uploads, kernels and
-// downloads of different jobs overlap:
-//   stage    uploads of every job; image plane, pulse table and range profiles of small jobs
-//            (nu * nv * npulses at most the fusion size)
-//   big[i]   image plane, pulse table, range profiles and accumulation of one larger job at a time
-//            (the least loaded of num_streams streams)
-//   fuse     accumulation of small jobs: staged small jobs gather while every launch-table slot is
-//            in flight, and when one is free, all of them (up to the table's capacity) are computed
-//            by one launch over a job-indexed tile list
+// Work flows through streams with one role each, so uploads, kernels and downloads of different
+// jobs overlap:
+//   stage    uploads of every job (copies only)
+//   prep     pulse table and range profiles of small jobs (nu * nv * npulses at most the fusion
+//            size)
+//   big[i]   pulse table, range profiles and accumulation of larger jobs, in row bands (the least
+//            loaded of num_streams streams)
+//   fuse     accumulation of small jobs: prepared small jobs gather while every launch-table slot
+//            is in flight, and when one is free, they are computed by one launch over a
+//            job-indexed tile list (up to the table's capacity and kFusePixelPulses)
 //   dl_small, dl_big   output downloads to the caller's buffers
-// A job's arena space returns to the free list when the scheduler sees its download complete.
+// A step that depends on another stream's work is enqueued by the scheduler once it has seen that
+// work complete (event query), never as a device-side wait: streams share the process's few
+// hardware queues, which run their packets in order, and a packet waiting there on work enqueued
+// later would hold every stream mapped to the same queue. A job's arena space returns to the free
+// list when the scheduler sees its download complete.
 //
 // A per-shape-class autotuner (shape class = N_fft, range-profile format, log2 of pulses and of
 // pixels) times the tile candidates on the first job of a class, over a band of its rows, and uses
@@ -51,6 +56,11 @@ constexpr int kFuseSlots = 2;                       // fused launches in flight 
 // kernels and copies are dispatched between bands.
 constexpr int64_t kBandTiles = 4096;
 constexpr int kBandsAhead = 2;
+// Pixel-pulses per fused launch at most (~6 ms of accumulation on an idle device), so that a fused
+// launch holds the hardware queue it shares with other streams no longer than a band does.
+constexpr int64_t kFusePixelPulses = 8'000'000'000LL;
+// Blocks of the range-profile launches of the engine (256 threads each, looping over pulses).
+constexpr int kSharedFftBlocks = 512;
 
 int64_t now_ns(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() -
@@ -143,6 +153,16 @@ class PinnedPool {
     std::map<size_t, std::vector<void *>> free_;
 };
 
+// Where an admitted job is: each step ends with an event the scheduler queries before the next.
+enum class Phase {
+    Uploading,    // copies on the stage stream, then ev_up
+    Prepping,     // small: pulse table and range profiles on the prep stream, then ev_ready
+    Ready,        // small: prepared, waiting in fuse_pending
+    Computing,    // accumulation in a fused launch (slot event) or in bands on a big stream
+    Finishing,    // large: all bands enqueued, then ev_acc
+    Downloading,  // output copy on a download stream, then ev_done
+};
+
 struct Job {
     uint64_t id = 0;
     sarbp_job d{};
@@ -160,9 +180,11 @@ struct Job {
     bool prepared = false;       // image plane, pulse table and range profiles enqueued
     bool done_recorded = false;  // ev_done is recorded (a never-recorded event queries as complete)
     int status = 1;              // 1 queued, 2 admitted, 0 done, < 0 error
+    Phase phase = Phase::Uploading;
     size_t off = SIZE_MAX;       // arena block
     uint8_t *dev = nullptr;
-    hipEvent_t ev_ready = nullptr;  // small: staged for the fused launch; large: uploaded
+    hipEvent_t ev_up = nullptr;     // inputs in device memory
+    hipEvent_t ev_ready = nullptr;  // small: prepared for the fused launch
     hipEvent_t ev_acc = nullptr;    // large: accumulated
     hipEvent_t ev_band[2] = {};     // large: completion of the outstanding accumulation bands
     bool banding = false;           // large: bands still to enqueue
@@ -180,6 +202,7 @@ struct FuseSlot {
     void *dev = nullptr;   // its device copy
     hipEvent_t ev = nullptr;
     bool busy = false;
+    std::vector<Job *> group;  // jobs of the launch in flight
 };
 
 struct Tuning {
@@ -198,14 +221,15 @@ struct sarbp_server {
     std::deque<std::shared_ptr<Job>> queue;
     std::vector<std::shared_ptr<Job>> inflight;
     std::map<uint64_t, std::shared_ptr<Job>> jobs;  // all jobs not yet collected by wait
-    std::vector<Job *> fuse_pending;                // small jobs staged, waiting for a fused launch
+    std::vector<Job *> fuse_pending;                // small jobs prepared, waiting for a fused launch
     FuseSlot slots[kFuseSlots];
     PinnedPool pinned;
     DeviceArena arena;
     uint8_t *arena_base = nullptr;
     std::map<std::tuple<int, int, int, int>, Tuning> tuning;
     std::vector<hipStream_t> streams;  // big[i]
-    hipStream_t stage = nullptr, fuse_stream = nullptr, dl_small = nullptr, dl_big = nullptr;
+    hipStream_t stage = nullptr, prep = nullptr, fuse_stream = nullptr, dl_small = nullptr,
+                dl_big = nullptr;
     std::vector<int64_t> stream_load;  // outstanding pixel-pulses per big stream
     uint64_t next_id = 1;
     int64_t used_bytes = 0;
@@ -272,7 +296,7 @@ int check_job(const sarbp_job *j) {
     return 0;
 }
 
-// Enqueue the uploads of an admitted job on the stage stream.
+// Enqueue the uploads of an admitted job on the stage stream, then its ev_up.
 int upload_job(sarbp_server *s, Job &j) {
     hipStream_t st = s->stage;
     const sarbp_job &d = j.d;
@@ -300,6 +324,7 @@ int upload_job(sarbp_server *s, Job &j) {
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess && d.pixel_locs)
         e = hipMemcpyAsync(b + j.ar.r, d.pixel_locs, 24 * npix, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(j.ev_up, st);
     if (e != hipSuccess) {
         set_error("job %llu: upload failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
         return halide_error_code_copy_to_device_failed;
@@ -328,7 +353,7 @@ int prepare_job(sarbp_server *s, Job &j, hipStream_t st) {
                                   (const double *)(b + j.ar.wp), d.nsamples, d.npulses, j.lay,
                                   s->tw[ilog2(n)], (float2 *)(b + j.ar.rot), b + j.ar.q, d.q_format,
                                   d.q_format == SARBP_Q_C16 ? (float *)(b + j.ar.qscale) : nullptr,
-                                  st);
+                                  st, kSharedFftBlocks);
     if (e != hipSuccess) {
         set_error("job %llu: range profiles failed: %s", (unsigned long long)j.id,
                   hipGetErrorString(e));
@@ -361,15 +386,16 @@ int prepare_job(sarbp_server *s, Job &j, hipStream_t st) {
     return 0;
 }
 
-// Output download after `after` and the completion event, on the job's download stream. On
-// failure nothing is recorded.
-int finish_job(sarbp_server *s, Job &j, hipEvent_t after) {
+// Output download of an accumulated job and its completion event, on the job's download stream.
+// On failure nothing is recorded.
+int finish_job(sarbp_server *s, Job &j) {
     hipStream_t st = j.small ? s->dl_small : s->dl_big;
     const size_t npix = (size_t)j.d.nu * (size_t)j.d.nv;
-    hipError_t e = hipStreamWaitEvent(st, after, 0);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(j.d.out, j.dev + j.ar.out, 16 * npix, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && (e = hipEventRecord(j.ev_done, st)) == hipSuccess) j.done_recorded = true;
+    hipError_t e = hipMemcpyAsync(j.d.out, j.dev + j.ar.out, 16 * npix, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && (e = hipEventRecord(j.ev_done, st)) == hipSuccess) {
+        j.done_recorded = true;
+        j.phase = Phase::Downloading;
+    }
     if (e != hipSuccess) {
         set_error("job %llu: download failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
         return halide_error_code_copy_to_host_failed;
@@ -395,7 +421,7 @@ int band_rows(const BpArgs &a) {
     return (int)std::max<int64_t>(1, kBandTiles / cols) * tile_height(a.ppt);
 }
 
-// Enqueue bands while fewer than kBandsAhead are outstanding; after the last band, the download.
+// Enqueue bands while fewer than kBandsAhead are outstanding; after the last band, ev_acc.
 int advance_bands(sarbp_server *s, Job &j) {
     hipStream_t st = s->streams[j.stream];
     while (j.bands_done < j.bands_enqueued &&
@@ -425,21 +451,18 @@ int advance_bands(sarbp_server *s, Job &j) {
                       hipGetErrorString(e));
             return halide_error_code_device_run_failed;
         }
-        return finish_job(s, j, j.ev_acc);
+        j.phase = Phase::Finishing;
     }
     return 0;
 }
 
-// Accumulate a job that runs alone; the first job of an undecided shape class first times every
-// tile candidate over a band of its top rows, which the full pass then rewrites.
+// Accumulate an uploaded job that runs alone; the first job of an undecided shape class first
+// times every tile candidate over a band of its top rows, which the full pass then rewrites.
 int run_alone(sarbp_server *s, Job &j) {
     hipStream_t st = s->streams[j.stream];
     Tuning &tu = s->tuning[j.cls];
-    hipError_t e = hipStreamWaitEvent(st, j.ev_ready, 0);
-    if (e != hipSuccess) {
-        set_error("job %llu: stream wait failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
-        return halide_error_code_device_run_failed;
-    }
+    hipError_t e = hipSuccess;
+    j.phase = Phase::Computing;
     if (!j.prepared) {
         int rv = prepare_job(s, j, st);
         if (rv) return rv;
@@ -484,8 +507,9 @@ size_t tiles_of(const Job &j, int th) {
     return (size_t)((j.d.nv + th - 1) / th) * (size_t)((j.d.nu + 15) / 16);
 }
 
-// One accumulation launch over the tiles of `group` (same format and tile shape, table within the
-// slot), using the free launch-table slot `slot`. Every job of the group is finished or failed.
+// One accumulation launch over the tiles of `group` (prepared jobs of one format and tile shape,
+// table within the slot), using the free launch-table slot `slot`; the jobs are downloaded once
+// the scheduler sees the slot's event complete. On failure every job of the group is failed.
 void run_fused(sarbp_server *s, FuseSlot &slot, std::vector<Job *> &group) {
     const int ppt = group[0]->ppt;
     const int th = tile_height(ppt);
@@ -502,12 +526,8 @@ void run_fused(sarbp_server *s, FuseSlot &slot, std::vector<Job *> &group) {
             for (int bx = 0; bx < j->d.nu; bx += 16) tiles[t++] = {(int)k, bx, by, 0};
     }
     hipStream_t fs = s->fuse_stream;
-    hipError_t e = hipSuccess;
-    for (Job *j : group)
-        if (e == hipSuccess) e = hipStreamWaitEvent(fs, j->ev_ready, 0);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(slot.dev, slot.host, ba + sizeof(FusedTile) * ntiles,
-                           hipMemcpyHostToDevice, fs);
+    hipError_t e = hipMemcpyAsync(slot.dev, slot.host, ba + sizeof(FusedTile) * ntiles,
+                                  hipMemcpyHostToDevice, fs);
     if (e == hipSuccess)
         e = launch_accumulate_fused((const BpArgs *)slot.dev,
                                     (const FusedTile *)((uint8_t *)slot.dev + ba), (int)ntiles,
@@ -519,10 +539,10 @@ void run_fused(sarbp_server *s, FuseSlot &slot, std::vector<Job *> &group) {
         return;
     }
     slot.busy = true;
+    slot.group = group;
     for (Job *j : group) {
         j->fused = true;
-        int r = finish_job(s, *j, slot.ev);
-        if (r) fail_inflight(s, *j, r);
+        j->phase = Phase::Computing;
     }
 }
 
@@ -533,6 +553,7 @@ void retire(sarbp_server *s, Job &j) {
     s->pinned.put(j.stage, j.stage_cls);
     j.stage = nullptr;
     (void)hipEventDestroy(j.ev_done);
+    (void)hipEventDestroy(j.ev_up);
     (void)hipEventDestroy(j.ev_ready);
     (void)hipEventDestroy(j.ev_acc);
     (void)hipEventDestroy(j.ev_begin);
@@ -540,15 +561,29 @@ void retire(sarbp_server *s, Job &j) {
         if (ev) (void)hipEventDestroy(ev);
 }
 
+// Pulse table and range profiles of an uploaded small job on the prep stream, then ev_ready.
+int prep_small(sarbp_server *s, Job &j) {
+    int rv = prepare_job(s, j, s->prep);
+    if (rv) return rv;
+    if (hipEventRecord(j.ev_ready, s->prep) != hipSuccess) {
+        set_error("job %llu: event record failed", (unsigned long long)j.id);
+        return halide_error_code_device_run_failed;
+    }
+    j.phase = Phase::Prepping;
+    return 0;
+}
+
+bool event_done(hipEvent_t ev) { return hipEventQuery(ev) == hipSuccess; }
+
 void scheduler(sarbp_server *s) {
     (void)hipSetDevice(s->cfg.device);
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
-        // 1. completions
+        // 1. completions, and the step after each completed step
         bool progressed = false;
         for (size_t i = 0; i < s->inflight.size();) {
             Job &j = *s->inflight[i];
-            if (j.done_recorded && hipEventQuery(j.ev_done) == hipSuccess) {
+            if (j.done_recorded && event_done(j.ev_done)) {
                 j.t_end = now_ns(s->t0);
                 if (j.status > 0) j.status = 0;
                 (void)hipEventElapsedTime(&j.device_ms, j.ev_begin, j.ev_done);
@@ -559,22 +594,36 @@ void scheduler(sarbp_server *s) {
                 ++i;
             }
         }
-        for (FuseSlot &sl : s->slots)
-            if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) sl.busy = false;
+        for (FuseSlot &sl : s->slots) {
+            if (!sl.busy || !event_done(sl.ev)) continue;
+            sl.busy = false;
+            for (Job *j : sl.group) {
+                int rv = finish_job(s, *j);
+                if (rv) fail_inflight(s, *j, rv);
+            }
+            sl.group.clear();
+        }
         for (auto &jp : s->inflight) {
             Job &j = *jp;
-            if (!j.banding) continue;
-            int rv = advance_bands(s, j);
-            if (rv) {
-                j.banding = false;
-                fail_inflight(s, j, rv);
+            if (j.done_recorded) continue;
+            int rv = 0;
+            if (j.phase == Phase::Uploading && event_done(j.ev_up)) {
+                rv = j.small ? prep_small(s, j) : run_alone(s, j);
+            } else if (j.phase == Phase::Prepping && event_done(j.ev_ready)) {
+                j.phase = Phase::Ready;
+                s->fuse_pending.push_back(&j);
+            } else if (j.phase == Phase::Computing && j.banding) {
+                rv = advance_bands(s, j);
+                if (rv) j.banding = false;
             }
+            if (rv == 0 && j.phase == Phase::Finishing && event_done(j.ev_acc)) rv = finish_job(s, j);
+            if (rv) fail_inflight(s, j, rv);
         }
         for (auto &kv : s->tuning) {
             Tuning &tu = kv.second;
             if (!tu.running) continue;
             bool done = true;
-            for (auto &pr : tu.probes) done = done && hipEventQuery(pr.second.second) == hipSuccess;
+            for (auto &pr : tu.probes) done = done && event_done(pr.second.second);
             if (!done) continue;
             float best_ms = 1e30f;
             for (auto &pr : tu.probes) {
@@ -620,28 +669,23 @@ void scheduler(sarbp_server *s) {
             j.stream = (int)(std::min_element(s->stream_load.begin(), s->stream_load.end()) -
                              s->stream_load.begin());
             if (!j.small) s->stream_load[j.stream] += j.pp;
+            (void)hipEventCreateWithFlags(&j.ev_up, hipEventDisableTiming);
             (void)hipEventCreateWithFlags(&j.ev_ready, hipEventDisableTiming);
             (void)hipEventCreateWithFlags(&j.ev_acc, hipEventDisableTiming);
             (void)hipEventCreate(&j.ev_done);
             (void)hipEventCreate(&j.ev_begin);
             (void)hipEventRecord(j.ev_begin, s->stage);
             s->inflight.push_back(sp);
+            j.phase = Phase::Uploading;
             int rv = upload_job(s, j);
-            if (rv == 0 && j.small) rv = prepare_job(s, j, s->stage);
-            if (rv == 0 && hipEventRecord(j.ev_ready, s->stage) != hipSuccess)
-                rv = halide_error_code_device_run_failed;
-            if (rv == 0) {
-                if (j.small) s->fuse_pending.push_back(&j);
-                else rv = run_alone(s, j);
-            }
             if (rv != 0) fail_inflight(s, j, rv);
             j.host_ns = now_ns(s->t0) - h0;
         }
 
-        // 3. small jobs: while every launch-table slot is in flight, staged jobs wait and gather;
+        // 3. small jobs: while every launch-table slot is in flight, prepared jobs wait and gather;
         // when a slot is free, the waiting jobs go into one launch per (format, tile shape, grid
-        // mode), as many as the slot's table holds. An idle GPU thus starts a small job at once and
-        // a busy one receives them in large batches.
+        // mode), as many as the slot's table and kFusePixelPulses allow. An idle GPU thus starts a
+        // small job at once and a busy one receives them in large batches.
         for (FuseSlot &sl : s->slots) {
             if (s->fuse_pending.empty()) break;
             if (sl.busy) continue;
@@ -650,12 +694,15 @@ void scheduler(sarbp_server *s) {
             const int th = tile_height(ppt);
             std::vector<Job *> group, rest;
             size_t bytes = 0;
+            int64_t pp = 0;
             for (Job *j : s->fuse_pending) {
                 const size_t need = sizeof(BpArgs) + 256 + sizeof(FusedTile) * tiles_of(*j, th);
                 if (j->d.q_format == fmt && j->ppt == ppt && j->args.g.on == grid &&
-                    (group.empty() || bytes + need <= kFuseSlotBytes)) {
+                    (group.empty() ||
+                     (bytes + need <= kFuseSlotBytes && pp + j->pp <= kFusePixelPulses))) {
                     group.push_back(j);
                     bytes += need;
+                    pp += j->pp;
                 } else {
                     rest.push_back(j);
                 }
@@ -689,7 +736,7 @@ void destroy_resources(sarbp_server *s) {
         (void)hipStreamDestroy(st);
     }
     s->streams.clear();
-    for (hipStream_t *st : {&s->stage, &s->fuse_stream, &s->dl_small, &s->dl_big}) {
+    for (hipStream_t *st : {&s->stage, &s->prep, &s->fuse_stream, &s->dl_small, &s->dl_big}) {
         if (!*st) continue;
         (void)hipStreamSynchronize(*st);
         (void)hipStreamDestroy(*st);
@@ -714,7 +761,7 @@ extern "C" void sarbp_server_config_init(sarbp_server_config *c) {
     c->max_inflight = 64;
     c->hbm_budget_bytes = (int64_t)64 << 30;
     c->fuse_max_pixel_pulses = (int64_t)1 << 31;
-    c->num_streams = 2;  // streams for large jobs, besides the stage, fuse and download streams
+    c->num_streams = 2;  // streams for large jobs, besides the stage, prep, fuse and download ones
     c->autotune = 1;
 }
 
@@ -750,7 +797,7 @@ extern "C" int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server 
         rv = halide_error_code_device_malloc_failed;
     }
     s->arena.reset((size_t)cfg->hbm_budget_bytes);
-    for (int i = 0; rv == 0 && i < cfg->num_streams + 4; ++i) {
+    for (int i = 0; rv == 0 && i < cfg->num_streams + 5; ++i) {
         hipStream_t st;
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
             set_error("sarbp_server_create: stream creation failed");
@@ -759,8 +806,9 @@ extern "C" int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server 
         }
         if (i < cfg->num_streams) s->streams.push_back(st);
         else if (i == cfg->num_streams) s->stage = st;
-        else if (i == cfg->num_streams + 1) s->fuse_stream = st;
-        else if (i == cfg->num_streams + 2) s->dl_small = st;
+        else if (i == cfg->num_streams + 1) s->prep = st;
+        else if (i == cfg->num_streams + 2) s->fuse_stream = st;
+        else if (i == cfg->num_streams + 3) s->dl_small = st;
         else s->dl_big = st;
     }
     for (FuseSlot &sl : s->slots) {

@@ -17,10 +17,26 @@ import time
 
 import numpy as np
 
+# Hardware queues the engine's streams want: num_streams + 5 streams and the null stream, each on
+# a queue of its own (HIP maps streams onto GPU_MAX_HW_QUEUES queues, 4 by default, and packets that
+# share a queue run in order).
+ENGINE_HW_QUEUES = 8
+
+
+def request_hw_queues(n: int = ENGINE_HW_QUEUES) -> None:
+    """Raise GPU_MAX_HW_QUEUES to n for this process; effective only before the HIP runtime starts
+    (the first HIP call of the process)."""
+    import os
+    cur = os.environ.get("GPU_MAX_HW_QUEUES", "")
+    if not cur.isdigit() or int(cur) < n:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(n)
+
+
 if __name__ == "__main__":  # run as a script: the package's parent directory goes on the path
     import os
     import sys
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    request_hw_queues()
 
 _pkg = importlib.import_module("halide-sar-app_amd")
 _scenes = importlib.import_module("halide-sar-app_amd.scenes")
@@ -320,28 +336,31 @@ def one_job_per_launch_config(device=0) -> dict:
 
 
 def benchmark(specs, load=0.7, seed_base=4000, device=0, log=print, **server_kw) -> dict:
-    """Serving figures of a job stream: capacity (all jobs submitted at once, after a warm-up pass
-    that also tunes every shape class), then a Poisson stream at `load` x that capacity, for the
-    serving engine and for the one-job-per-launch configuration at the same arrival rate."""
+    """Serving figures of a job stream for the serving engine and for the one-job-per-launch
+    configuration: each one's capacity (all jobs submitted at once, after a warm-up pass that also
+    tunes every shape class), then two Poisson streams run on both: `stream` at `load` x the
+    engine's capacity and `stream_low` at `load` x the one-job-per-launch capacity (a rate both
+    sustain)."""
     t = time.perf_counter()
     factory = JobFactory(specs, seed_base=seed_base, device=device)
     log(f"serving: {len(specs)} jobs generated in {time.perf_counter() - t:.1f} s")
     res = {}
-    with Server(device=device, **server_kw) as srv:
-        replay(srv, factory, None)
-        cap = summarize(*replay(srv, factory, None)[:2])
-        rate = load * cap["jobs_per_s"]
-        log(f"serving: capacity {cap['jobs_per_s']:.1f} jobs/s; Poisson stream at {rate:.1f}/s")
-        res["serving"] = dict(capacity=cap, stream=summarize(*replay(srv, factory, rate)[:2]))
-    log(f"serving: {res['serving']['stream']}")
-    with Server(**one_job_per_launch_config(device)) as srv:
-        replay(srv, factory, None)
-        cap_b = summarize(*replay(srv, factory, None)[:2])
-        log(f"one-job-per-launch: capacity {cap_b['jobs_per_s']:.1f} jobs/s")
-        res["one_job_per_launch"] = dict(capacity=cap_b,
-                                         stream=summarize(*replay(srv, factory, rate)[:2]))
-    log(f"one-job-per-launch: {res['one_job_per_launch']['stream']}")
+    with Server(device=device, **server_kw) as srv, \
+            Server(**one_job_per_launch_config(device)) as base:
+        caps = {}
+        for name, s in (("serving", srv), ("one_job_per_launch", base)):
+            replay(s, factory, None)
+            caps[name] = summarize(*replay(s, factory, None)[:2])
+            log(f"{name}: capacity {caps[name]['jobs_per_s']:.1f} jobs/s")
+        rate = load * caps["serving"]["jobs_per_s"]
+        rate_low = load * caps["one_job_per_launch"]["jobs_per_s"]
+        for name, s in (("serving", srv), ("one_job_per_launch", base)):
+            res[name] = dict(capacity=caps[name])
+            for key, r in (("stream", rate), ("stream_low", rate_low)):
+                res[name][key] = summarize(*replay(s, factory, r)[:2])
+                log(f"{name}: Poisson {r:.1f}/s: {res[name][key]}")
     res["arrival_rate_hz"] = rate
+    res["arrival_rate_low_hz"] = rate_low
     res["load"] = load
     return res
 

@@ -123,7 +123,11 @@ typedef struct sarbp_plan_desc {
     int32_t device;                  /* HIP device ordinal */
     int32_t tile_ppt;                /* pixels per thread of the accumulation tile (2, 3 or 4;
                                         0 picks the default) */
-    int32_t reserved[6];
+    int32_t fft_blocks;              /* range profiles: 0 = one 512-thread block per profile;
+                                        > 0 = at most this many 256-thread blocks, each looping
+                                        over profiles (N_fft <= 8192; for a device shared with
+                                        other streams' accumulation launches). Same values. */
+    int32_t reserved[5];
 } sarbp_plan_desc;
 
 void sarbp_plan_desc_init(sarbp_plan_desc *d);
@@ -184,7 +188,12 @@ int sarbp_sim_phs_device(const double *pos_dev, int32_t npulses, const double *k
  * several streams so that transfers and kernels of different jobs overlap, and jobs at or below
  * `fuse_max_pixel_pulses` share one accumulation launch over a job-indexed tile list. A job's result
  * is the same whether it runs alone or fused (same per-pixel code). Host buffers of a job must stay
- * valid until sarbp_server_wait returns for it; page-locked buffers let the copies overlap. */
+ * valid until sarbp_server_wait returns for it; page-locked buffers let the copies overlap.
+ * The engine uses num_streams + 5 streams (uploads, small-job preparation, fused launches, two
+ * download streams). HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless
+ * the environment sets it before the runtime starts), and packets sharing a queue run in order, so
+ * with fewer queues than streams small jobs can wait behind large jobs' bands: results are the
+ * same, latency is lower with num_streams + 6 queues (8 for the default, counting the null stream). */
 typedef struct sarbp_server sarbp_server;
 
 typedef struct sarbp_server_config {
@@ -192,7 +201,7 @@ typedef struct sarbp_server_config {
     int32_t max_inflight;           /* admitted jobs at once (>= 1) */
     int64_t hbm_budget_bytes;       /* device memory the admitted jobs may hold together */
     int64_t fuse_max_pixel_pulses;  /* jobs with nu*nv*npulses <= this are fused (0: never) */
-    int32_t num_streams;            /* job streams, 1..16 */
+    int32_t num_streams;            /* large-job streams, 1..16 (default 2) */
     int32_t autotune;               /* 1: time the tile candidates on a shape class's first job */
     int32_t reserved[8];
 } sarbp_server_config;

@@ -164,6 +164,35 @@ def test_range_profiles_vs_oracle(sar, orc, scenes, torch_dev):
         assert err < 1e-5
 
 
+@pytest.mark.parametrize("q_format", [0, 1])
+def test_range_profiles_shared_kernel_same_bits(sar, orc, scenes, torch_dev, q_format):
+    """The 256-thread range-profile variant (fft_blocks > 0, used by the serving engine) writes the
+    same stored rows, bit for bit, as the one-block-per-profile kernel, for any block count (each
+    block loops over profiles), and N_fft above its 8192 limit falls back to the latter."""
+    torch = torch_dev
+    for np_, ns, n_fft in ((37, 300, 512), (301, 1000, 2048), (64, 2048, 8192), (9, 5000, 16384)):
+        sc = scenes.tiny_scene(npulses=np_, nsamples=ns, seed=np_ + n_fft)
+        phs = torch.from_numpy(sc.phs).cuda()
+        k_r = torch.from_numpy(sc.k_r).cuda()
+        rows = {}
+        for blocks in (0, 1, 7, 512):
+            plan = sar.Plan(sc.nsamples, sc.npulses, n_fft, sc.taylor, sc.delta_r, 8, 8,
+                            q_format=q_format, fft_blocks=blocks)
+            q = torch.full((plan.q_bytes,), 0x5A, dtype=torch.uint8, device="cuda")
+            print(f"np={np_} N={n_fft} format={q_format} fft_blocks={blocks}", flush=True)
+            plan.range_profiles(phs, k_r, q=q)
+            torch.cuda.synchronize()
+            rows[blocks] = q.cpu().numpy()
+        for blocks in (1, 7, 512):
+            assert np.array_equal(rows[blocks], rows[0]), (np_, n_fft, blocks)
+        plan.range_profiles(phs, k_r)
+        got = plan.read_profiles()
+        ref = orc.range_profiles(sc.phs, sc.k_r, sc.taylor, n_fft)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        print(f"   Q vs oracle: max err / max |Q| = {err:.3e}")
+        assert err < (1e-5 if q_format == 0 else 2e-3)
+
+
 def test_c2_full_size_sampled_and_scaling(sar, orc, scenes, pipe, torch_dev):
     """Configuration C2 at full size (2048^2 x 2000 pulses, N 8192): oracle on 9216 sampled pixels
     (two full rows, two full columns, 1024 random ones) plus exact 2x linearity of the whole image."""

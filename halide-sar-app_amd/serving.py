@@ -23,12 +23,12 @@ import numpy as np
 ENGINE_HW_QUEUES = 8
 
 
-def request_hw_queues(n: int = ENGINE_HW_QUEUES) -> None:
-    """Raise GPU_MAX_HW_QUEUES to n for this process; effective only before the HIP runtime starts
-    (the first HIP call of the process)."""
+def request_hw_queues(n: int = ENGINE_HW_QUEUES, exact: bool = False) -> None:
+    """Raise GPU_MAX_HW_QUEUES to n for this process (set it to n with exact=True); effective only
+    before the HIP runtime starts (the first HIP call of the process)."""
     import os
     cur = os.environ.get("GPU_MAX_HW_QUEUES", "")
-    if not cur.isdigit() or int(cur) < n:
+    if exact or not cur.isdigit() or int(cur) < n:
         os.environ["GPU_MAX_HW_QUEUES"] = str(n)
 
 
@@ -36,7 +36,9 @@ if __name__ == "__main__":  # run as a script: the package's parent directory go
     import os
     import sys
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-    request_hw_queues()
+    # --hw-queues is applied here, before anything below can start the HIP runtime
+    _hwq = [sys.argv[k + 1] for k, a in enumerate(sys.argv[:-1]) if a == "--hw-queues"]
+    request_hw_queues(int(_hwq[-1]) if _hwq else ENGINE_HW_QUEUES, exact=bool(_hwq))
 
 _pkg = importlib.import_module("halide-sar-app_amd")
 _scenes = importlib.import_module("halide-sar-app_amd.scenes")
@@ -377,6 +379,8 @@ def main(argv=None):
     ap.add_argument("--streams", type=int, default=2, help="large-job streams of the engine")
     ap.add_argument("--fuse-max", type=float, default=2 ** 31,
                     help="largest job (pixel-pulses) the engine fuses")
+    ap.add_argument("--hw-queues", type=int, default=ENGINE_HW_QUEUES,
+                    help="GPU_MAX_HW_QUEUES for this process (HIP's default is 4)")
     a = ap.parse_args(argv)
     specs = c4_specs() if a.config == "C4" else c5_specs()
     if a.jobs:

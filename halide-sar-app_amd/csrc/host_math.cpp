@@ -54,7 +54,7 @@ RowLayout row_layout(int ns, int n_fft, double delta_r) {
 }
 
 PixelGrid make_grid(int nu, int nv, double d_u, double d_v, const double *u_hat,
-                    const double *v_hat, double *step) {
+                    const double *v_hat) {
     PixelGrid g{};
     g.on = 1;
     g.u0 = floordiv(-nu, 2);  // img_plane.cpp:12-14
@@ -64,8 +64,6 @@ PixelGrid make_grid(int nu, int nv, double d_u, double d_v, const double *u_hat,
     for (int c = 0; c < 3; ++c) {
         g.uh[c] = u_hat[c];
         g.vh[c] = v_hat[c];
-        // output row y + 16 is grid row vy - 16
-        if (step) step[c] = v_hat[c] * (-16.0 * d_v);
     }
     return g;
 }

@@ -13,14 +13,15 @@ namespace sarbp {
 // Per-pulse constants of the accumulation, computed on the device from pos and k_r.
 // With d = |r - pos_p| (f64) the row coordinate of the interpolation is t = B - inv_lsr * d, and
 // |r - pos_p|^2 = |r|^2 + C - 2 r.pos_p.
+// Squared ranges are taken in row units: sigma |r - pos_p|^2 with sigma = inv_lsr^2.
 struct PulseConst {
     double px, py, pz;  // platform position, f32 widened
-    double C;           // |pos_p|^2
+    double Cs;          // sigma |pos_p|^2
     double B;           // (nr0 - lsa) * inv_lsr + ext
     double nr0;         // ||pos_p|| under the selected policy, widened
-    double S;           // 2 step.pos_p for the grid's 16-row step (0 without a grid): in grid mode
-                        // |r + k step - pos|^2 = |r - pos|^2 + (|r + k step|^2 - |r|^2) - k S
-    double pad1;
+    double U, V;        // grid mode: -2 sigma u_hat.pos_p, -2 sigma v_hat.pos_p, so that for
+                        // r = u u_hat + v v_hat: sigma |r - pos_p|^2 = sigma |r|^2 + Cs + u U + v V
+                        // (0 without a grid)
 };
 
 // A pixel grid generated in the accumulation kernel instead of read from pixel_locs
@@ -32,10 +33,9 @@ struct PixelGrid {
     double d_u, d_v;
     double uh[3], vh[3];
 };
-// The grid of an image nu x nv with spacing d_u, d_v and unit vectors u_hat, v_hat; step receives
-// the location change of 16 output rows (the rows one accumulation thread holds apart).
+// The grid of an image nu x nv with spacing d_u, d_v and unit vectors u_hat, v_hat.
 PixelGrid make_grid(int nu, int nv, double d_u, double d_v, const double *u_hat,
-                    const double *v_hat, double *step);
+                    const double *v_hat);
 
 // Launch-wide constants written next to the pulse table.
 struct BpHeader {
@@ -70,9 +70,9 @@ void set_error(const char *fmt, ...);
 int taylor_window(int num, int s_l, int den_policy, double *out);
 
 // Kernel launchers (kernels.hip). All are asynchronous on `stream`; return hipError_t.
-// step: the grid's 16-row step (3 doubles, host memory) or NULL.
+// grid: the pixel grid of grid mode (host memory) or NULL.
 hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np,
-                               const RowLayout &lay, int norm_policy, const double *step,
+                               const RowLayout &lay, int norm_policy, const PixelGrid *grid,
                                PulseConst *pc, BpHeader *hdr, hipStream_t stream);
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
                                  const double *wp, int ns, int np, const RowLayout &lay,

@@ -23,6 +23,19 @@
 
 #include "internal.h"
 
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// Structured buffer loads (the LLVM intrinsic; clang has no builtin for it): the address is
+// base + vindex * stride + voffset, the multiply done by the load unit (buffer_load ... idxen)
+// instead of a VALU shift per element.
+__device__ u32x4 struct_load_b128(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset,
+                                  int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v4i32");
+__device__ u32x2 struct_load_b64(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset,
+                                 int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v2i32");
+
 namespace sarbp {
 
 namespace {
@@ -58,24 +71,24 @@ __device__ inline void storeq(QPairF16 *p, float2 q, float2 dq) {
     v.dq = __float22half2_rn(dq);
     *p = v;
 }
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
-// Buffer resource over one padded row (raw buffer, 32-bit data format word as for gfx9 raw loads):
-// reads at byte offsets past num_records return 0 instead of touching memory.
-__device__ inline __amdgpu_buffer_rsrc_t row_rsrc(const void *row, unsigned bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(row), (short)0, (int)bytes,
+// Buffer resource over one padded row of `len` elements of `stride` bytes (32-bit data format
+// word as for gfx9 buffer loads). Every index the accumulation forms lies in [0, len): the fast path
+// runs only for tiles whose pixels are all within the rows' cover radius (lanes outside the image
+// take an in-image location) and the exact path clamps.
+__device__ inline __amdgpu_buffer_rsrc_t row_rsrc(const void *row, int len, int stride) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(row), (short)stride, len,
                                              0x00020000);
 }
-__device__ inline float4 loadq(const QPairF32 *, __amdgpu_buffer_rsrc_t r, unsigned idx) {
-    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, idx * 16u, 0, 0);
+__device__ inline float4 loadq(const QPairF32 *, __amdgpu_buffer_rsrc_t r, int idx) {
+    const u32x4 v = struct_load_b128(r, idx, 0, 0, 0);
     const f32x4 f = __builtin_bit_cast(f32x4, v);
     return make_float4(f.x, f.y, f.z, f.w);
 }
-__device__ inline float4 loadq(const QPairF16 *, __amdgpu_buffer_rsrc_t r, unsigned idx) {
-    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8u, 0, 0);
+__device__ inline float4 loadq(const QPairF16 *, __amdgpu_buffer_rsrc_t r, int idx) {
+    const u32x2 v = struct_load_b64(r, idx, 0, 0, 0);
     const f16x4 h = __builtin_bit_cast(f16x4, v);
     return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
 }
@@ -94,14 +107,16 @@ __device__ inline int trunc_clamp(double t, int hi) {
     return i;
 }
 
-// sqrt(s) from the hardware reciprocal square root (about 23 correct bits) and one
-// Newton-Raphson step in f64: relative error ~1.5 * (2^-23)^2, below 1e-13.
-__device__ inline double fast_sqrt(double s) {
-    double y = __builtin_amdgcn_rsq(s);
-    double g = s * y;
-    double h = 0.5 * y;
-    double r = __builtin_fma(-g, h, 0.5);
-    return __builtin_fma(g, r, g);
+// Row coordinate t = B - sqrt(s) for a squared range s in row units (sigma |r - pos|^2,
+// sigma = inv_lsr^2, so sqrt(s) = d / lsr): the hardware reciprocal square root y (about 23
+// correct bits) and one Newton-Raphson step in f64, a = s y, sqrt(s) = a (3 - a y) / 2, with the
+// subtraction from B in the last fused multiply-add. Relative error of sqrt(s) ~1.5 (2^-23)^2.
+__device__ inline double row_coord(double s, double B) {
+    const double y = __builtin_amdgcn_rsq(s);
+    const double a = s * y;
+    const double h = a * y;
+    const double c = __builtin_fma(-0.5, h, 1.5);
+    return __builtin_fma(-a, c, B);
 }
 
 // One component of a grid pixel location, rounded as img_plane.cpp:77-113 computes it (product,
@@ -117,8 +132,9 @@ __device__ inline double grid_coord(double uh, double u, double vh, double v) {
 
 __global__ void pulse_const_kernel(const float *__restrict__ pos, const float *__restrict__ k_r,
                                    int ns, int np, double lsa, double lsr, double inv_lsr, int ext,
-                                   int policy, double sx, double sy, double sz,
-                                   PulseConst *__restrict__ pc, BpHeader *__restrict__ hdr) {
+                                   int policy, double ux, double uy, double uz, double vx,
+                                   double vy, double vz, PulseConst *__restrict__ pc,
+                                   BpHeader *__restrict__ hdr) {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
@@ -151,12 +167,14 @@ __global__ void pulse_const_kernel(const float *__restrict__ pos, const float *_
     c.px = x;
     c.py = y;
     c.pz = z;
-    c.C = c.px * c.px + c.py * c.py + c.pz * c.pz;
+    // squared ranges in row units: sigma |r - pos|^2 = sigma |r|^2 + Cs - 2 sigma r.pos
+    const double sigma = inv_lsr * inv_lsr;
+    c.Cs = sigma * (c.px * c.px + c.py * c.py + c.pz * c.pz);
     // t = luts + ext with luts = (nr0 - d - lsa) / lsr (backprojection.cpp:53-54,117,120)
     c.B = (nr0 - lsa) * inv_lsr + (double)ext;
     c.nr0 = nr0;
-    c.S = 2.0 * (sx * c.px + sy * c.py + sz * c.pz);
-    c.pad1 = 0.0;
+    c.U = (-2.0 * sigma) * (ux * c.px + uy * c.py + uz * c.pz);
+    c.V = (-2.0 * sigma) * (vx * c.px + vy * c.py + vz * c.pz);
     pc[p] = c;
 }
 
@@ -409,30 +427,39 @@ constexpr int kBpPulseBlock = SARBP_BP_PULSE_BLOCK;
 constexpr int kDefaultPpt = 2;
 
 // Tile of 16 x (16 * PPT) output pixels per block; thread (tx, ty) owns pixels (tx, ty + 16 j).
-// Each pixel keeps -2 r and |r|^2 in f64 registers; per pulse p (constants from the pulse table):
-//   d  = sqrt(|r|^2 + |pos_p|^2 - 2 r.pos_p)          f64: v_rsq_f64 + one Newton step
-//   t  = B_p - d / lsr                                f64 row coordinate
-//   j  = floor(t), w = t - j                          int, f64 -> f32
-//   term = (A_j + w B_j) * exp(2 pi i kappa w)        one 16-byte gather; f32
-//   acc += term                                       f32, flushed to f64 every 64 pulses
+// Squared ranges are kept in row units, s = sigma |r - pos_p|^2 with sigma = inv_lsr^2, so that
+// sqrt(s) is d / lsr. Per pulse p (constants from the pulse table):
+//   s  = sigma |r|^2 + Cs_p - 2 sigma r.pos_p          f64, from per-pixel terms (PixTerms)
+//   t  = B_p - sqrt(s)                                 f64: v_rsq_f64 + one Newton step (row_coord)
+//   j  = floor(t), w = t - j                           int, f64 -> f32
+//   term = (A_j + w B_j) * exp(2 pi i kappa w)         one 16-byte gather; f32
+//   acc += term                                        f32, flushed to f64 every 64 pulses
 // This fast path needs every t of the block inside the row; a block whose pixels reach beyond the
 // row's cover radius takes the exact path instead: j clamped to the row, the phase
 // kappa (t - j) reduced in f64 (the clamped edge elements have B = 0, so the weight is moot).
-// GRID: only pixel 0's (m2x, m2y, m2z, r2) are used; pixel j > 0 of the thread lies j grid steps
-// (16 output rows) further and its |r - pos|^2 follows from pixel 0's as s2_0 + dr2[j] - j S_p.
+//
+// Per-pixel terms, in registers for the whole pulse loop:
+//   r mode:    s_j = fma(a_j, px, fma(b_j, py, fma(c_j, pz, r2_j + Cs)))  with (a, b, c) = -2 sigma r
+//   grid mode: r(u, v) = u u_hat + v v_hat, so -2 sigma r.pos = u U_p + v V_p (pulse table) and
+//              s_0 = fma(u, U, fma(v, V, r2_0 + Cs))  with a_0 = u, b_0 = v of the thread's pixel 0;
+//              pixel j lies 16 j output rows further: s_j = s_0 + fma(c_j, V, dr2_j) with
+//              c_j = v_j - v_0 and dr2_j = r2_j - r2_0
+// (r2 = sigma |r|^2). Lanes whose pixel is outside the region take pixel 0's location (grid) or the
+// origin, so that their loads stay inside the row.
+template <int PPT>
+struct PixTerms {
+    double a[PPT], b[PPT], c[PPT], r2[PPT], dr2[PPT];
+};
+
 template <typename QT, int PPT, bool FAST, bool GRID>
-__device__ inline void accumulate_pulses(const BpArgs &a, const double (&m2x)[PPT],
-                                         const double (&m2y)[PPT], const double (&m2z)[PPT],
-                                         const double (&r2)[PPT], const double (&dr2)[PPT],
+__device__ inline void accumulate_pulses(const BpArgs &a, const PixTerms<PPT> &P,
                                          double (&accr)[PPT], double (&acci)[PPT]) {
     constexpr bool kScaled = sizeof(QT) == sizeof(QPairF16);
-    const double ninv = -a.hdr->inv_lsr;
     const double kappa = a.hdr->kappa;
     const float kappa_f = (float)kappa;
     const int np = a.np;
     const int jhi = a.len - 2;
     const QT *qbase = reinterpret_cast<const QT *>(a.q);
-    const unsigned row_bytes = (unsigned)((size_t)a.len * sizeof(QT));
     for (int p0 = 0; p0 < np; p0 += kBpPulseBlock) {
         float sr[PPT], si[PPT];
 #pragma unroll
@@ -441,24 +468,26 @@ __device__ inline void accumulate_pulses(const BpArgs &a, const double (&m2x)[PP
 #pragma unroll SARBP_BP_UNROLL
         for (int p = p0; p < pe; ++p) {
             const PulseConst c = a.pc[p];
-            const __amdgpu_buffer_rsrc_t row = row_rsrc(qbase + (size_t)p * a.len, row_bytes);
+            const __amdgpu_buffer_rsrc_t row =
+                row_rsrc(qbase + (size_t)p * a.len, a.len, (int)sizeof(QT));
             const float qs = kScaled ? a.qscale[p] : 1.0f;
             int ii[PPT];
             float ww[PPT], xx[PPT];
-            const double s20 = __builtin_fma(
-                m2x[0], c.px, __builtin_fma(m2y[0], c.py, __builtin_fma(m2z[0], c.pz, r2[0] + c.C)));
+            double sq[PPT];
+            if (GRID) {
+                sq[0] = __builtin_fma(P.a[0], c.U, __builtin_fma(P.b[0], c.V, P.r2[0] + c.Cs));
+#pragma unroll
+                for (int j = 1; j < PPT; ++j) sq[j] = sq[0] + __builtin_fma(P.c[j], c.V, P.dr2[j]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < PPT; ++j)
+                    sq[j] = __builtin_fma(P.a[j], c.px,
+                                          __builtin_fma(P.b[j], c.py,
+                                                        __builtin_fma(P.c[j], c.pz, P.r2[j] + c.Cs)));
+            }
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
-                double s2;
-                if (j == 0)
-                    s2 = s20;
-                else if (GRID)
-                    s2 = s20 + __builtin_fma(-(double)j, c.S, dr2[j]);
-                else
-                    s2 = __builtin_fma(m2x[j], c.px,
-                                       __builtin_fma(m2y[j], c.py, __builtin_fma(m2z[j], c.pz, r2[j] + c.C)));
-                const double d = fast_sqrt(s2);
-                const double t = __builtin_fma(ninv, d, c.B);
+                const double t = row_coord(sq[j], c.B);
                 if (FAST) {
                     ii[j] = trunc_i32(t);
                     ww[j] = (float)__builtin_amdgcn_fract(t);
@@ -472,7 +501,7 @@ __device__ inline void accumulate_pulses(const BpArgs &a, const double (&m2x)[PP
             }
             float4 e[PPT];
 #pragma unroll
-            for (int j = 0; j < PPT; ++j) e[j] = loadq(qbase, row, (unsigned)ii[j]);
+            for (int j = 0; j < PPT; ++j) e[j] = loadq(qbase, row, ii[j]);
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
                 const float sn = __builtin_amdgcn_sinf(xx[j]);
@@ -502,15 +531,16 @@ __device__ inline void accumulate_pulses(const BpArgs &a, const double (&m2x)[PP
 // xy_to_pixel = nu * (nv - y - 1) + x, backprojection.cpp:130; r is constant_exterior 0).
 template <bool GRID>
 __device__ inline void pixel_loc(const BpArgs &a, int lx, int ly, bool ok, double &rx, double &ry,
-                                 double &rz) {
+                                 double &rz, double &u, double &v) {
     const int x = a.x0 + lx, y = a.y0 + ly;
+    u = v = 0.0;
     if (GRID) {
-        // grid regions lie inside the image (sarbp_plan_set_grid checks), so a thread's pixels are
-        // whole grid steps apart; lanes past the region's edge take location 0, as with r
+        // grid regions lie inside the image (sarbp_plan_set_grid checks); lanes past the region's
+        // edge take location 0, as with r
         rx = ry = rz = 0.0;
         if (!ok) return;
-        const double u = (double)(a.g.u0 + x) * a.g.d_u;
-        const double v = (double)(a.g.v0 + (a.nv - y - 1)) * a.g.d_v;
+        u = (double)(a.g.u0 + x) * a.g.d_u;
+        v = (double)(a.g.v0 + (a.nv - y - 1)) * a.g.d_v;
         rx = grid_coord(a.g.uh[0], u, a.g.vh[0], v);
         ry = grid_coord(a.g.uh[1], u, a.g.vh[1], v);
         rz = grid_coord(a.g.uh[2], u, a.g.vh[2], v);
@@ -532,21 +562,37 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, int
     if (threadIdx.x == 0) far_pixels = 0;
     __syncthreads();
 
-    double m2x[PPT], m2y[PPT], m2z[PPT], r2[PPT], dr2[PPT];
+    const double inv_lsr = a.hdr->inv_lsr;
+    const double sigma = inv_lsr * inv_lsr;
+    PixTerms<PPT> P;
     bool ok[PPT];
     bool far = false;
+    double v0 = 0.0;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         const int lx = bx + tx, ly = by + ty + 16 * j;
         ok[j] = lx < a.width && ly < a.height;
-        double rx, ry, rz;
-        pixel_loc<GRID>(a, lx, ly, ok[j], rx, ry, rz);
-        m2x[j] = -2.0 * rx;
-        m2y[j] = -2.0 * ry;
-        m2z[j] = -2.0 * rz;
-        r2[j] = rx * rx + ry * ry + rz * rz;
-        dr2[j] = r2[j] - r2[0];
-        far = far || !(r2[j] <= a.r_cover2);
+        double rx, ry, rz, u, v;
+        pixel_loc<GRID>(a, lx, ly, ok[j], rx, ry, rz, u, v);
+        const double rr = rx * rx + ry * ry + rz * rz;
+        far = far || !(rr <= a.r_cover2);
+        const double r2s = sigma * rr;
+        if (GRID) {
+            if (j == 0) {
+                P.a[0] = u;
+                P.b[0] = v;
+                P.r2[0] = r2s;
+                v0 = v;
+            }
+            // a pixel outside the region computes pixel 0's range
+            P.c[j] = ok[j] && ok[0] ? v - v0 : 0.0;
+            P.dr2[j] = ok[j] && ok[0] ? r2s - P.r2[0] : 0.0;
+        } else {
+            P.a[j] = (-2.0 * sigma) * rx;
+            P.b[j] = (-2.0 * sigma) * ry;
+            P.c[j] = (-2.0 * sigma) * rz;
+            P.r2[j] = r2s;
+        }
     }
     if (far) far_pixels = 1;
     __syncthreads();
@@ -555,9 +601,9 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, int
 #pragma unroll
     for (int j = 0; j < PPT; ++j) accr[j] = acci[j] = 0.0;
     if (far_pixels == 0)
-        accumulate_pulses<QT, PPT, true, GRID>(a, m2x, m2y, m2z, r2, dr2, accr, acci);
+        accumulate_pulses<QT, PPT, true, GRID>(a, P, accr, acci);
     else
-        accumulate_pulses<QT, PPT, false, GRID>(a, m2x, m2y, m2z, r2, dr2, accr, acci);
+        accumulate_pulses<QT, PPT, false, GRID>(a, P, accr, acci);
 
     // fimg = img * expj(k_c * dr_i(pixel, npulses / 2)) (backprojection.cpp:135)
     const PulseConst cm = a.pc[a.np / 2];
@@ -566,14 +612,8 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, int
     for (int j = 0; j < PPT; ++j) {
         if (!ok[j]) continue;
         const int lx = bx + tx, ly = by + ty + 16 * j;
-        double rx, ry, rz;
-        if (GRID) {
-            pixel_loc<true>(a, lx, ly, true, rx, ry, rz);
-        } else {
-            rx = -0.5 * m2x[j];
-            ry = -0.5 * m2y[j];
-            rz = -0.5 * m2z[j];
-        }
+        double rx, ry, rz, u, v;
+        pixel_loc<GRID>(a, lx, ly, true, rx, ry, rz, u, v);
         const double dx = rx - cm.px, dy = ry - cm.py, dz = rz - cm.pz;
         const double dm = sqrt(dx * dx + dy * dy + dz * dz);
         const double ang = k_c * (cm.nr0 - dm);
@@ -736,14 +776,16 @@ __global__ __launch_bounds__(kSimThreads) void sim_phs_kernel(
 // ------------------------------------------------------------------ launchers
 
 hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np,
-                               const RowLayout &lay, int norm_policy, const double *step,
+                               const RowLayout &lay, int norm_policy, const PixelGrid *grid,
                                PulseConst *pc, BpHeader *hdr, hipStream_t stream) {
     const int threads = 256;
     const int blocks = (np + threads - 1) / threads;
-    const double sx = step ? step[0] : 0.0, sy = step ? step[1] : 0.0, sz = step ? step[2] : 0.0;
+    const bool g = grid && grid->on;
     hipLaunchKernelGGL(pulse_const_kernel, dim3(blocks > 0 ? blocks : 1), dim3(threads), 0, stream,
-                       pos, k_r, ns, np, lay.lsa, lay.lsr, lay.inv_lsr, lay.ext, norm_policy, sx,
-                       sy, sz, pc, hdr);
+                       pos, k_r, ns, np, lay.lsa, lay.lsr, lay.inv_lsr, lay.ext, norm_policy,
+                       g ? grid->uh[0] : 0.0, g ? grid->uh[1] : 0.0, g ? grid->uh[2] : 0.0,
+                       g ? grid->vh[0] : 0.0, g ? grid->vh[1] : 0.0, g ? grid->vh[2] : 0.0, pc,
+                       hdr);
     return hipGetLastError();
 }
 

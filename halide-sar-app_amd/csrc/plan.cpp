@@ -22,7 +22,6 @@ struct sarbp_plan {
     BpHeader *hdr = nullptr;    // launch-wide constants
     const float2 *tw = nullptr; // twiddles exp(-2 pi i t / N)
     PixelGrid grid{};           // grid.on: pixel locations generated in the accumulation kernel
-    double step[3] = {0, 0, 0}; // the grid's 16-row step
 };
 
 namespace {
@@ -325,7 +324,7 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = launch_pulse_consts((const float *)pos_dev, (const float *)k_r_dev, d.nsamples,
                                        d.npulses, p->lay, d.norm_r0_policy,
-                                       p->grid.on ? p->step : nullptr, p->pc, p->hdr, s);
+                                       p->grid.on ? &p->grid : nullptr, p->pc, p->hdr, s);
     if (e == hipSuccess) {
         BpArgs a;
         a.q = q_dev ? q_dev : p->q;
@@ -367,7 +366,6 @@ extern "C" int sarbp_plan_set_grid(sarbp_plan *p, double d_u, double d_v, const 
     }
     if (!u_hat || !v_hat) {  // back to pixel locations read from r
         p->grid = PixelGrid{};
-        p->step[0] = p->step[1] = p->step[2] = 0.0;
         return 0;
     }
     const sarbp_plan_desc &d = p->d;
@@ -375,7 +373,7 @@ extern "C" int sarbp_plan_set_grid(sarbp_plan *p, double d_u, double d_v, const 
         set_error("sarbp_plan_set_grid: the plan's output region must lie inside the nu x nv image");
         return halide_error_code_constraint_violated;
     }
-    p->grid = make_grid(d.nu, d.nv, d_u, d_v, u_hat, v_hat, p->step);
+    p->grid = make_grid(d.nu, d.nv, d_u, d_v, u_hat, v_hat);
     return 0;
 }
 
@@ -397,7 +395,7 @@ int plan_prepare(sarbp_plan *p, const void *phs_dev, const void *k_r_dev, const 
     const sarbp_plan_desc &d = p->d;
     hipError_t e = launch_pulse_consts((const float *)pos_dev, (const float *)k_r_dev, d.nsamples,
                                        d.npulses, p->lay, d.norm_r0_policy,
-                                       p->grid.on ? p->step : nullptr, p->pc, p->hdr, stream);
+                                       p->grid.on ? &p->grid : nullptr, p->pc, p->hdr, stream);
     if (e != hipSuccess) {
         set_error("pulse-table launch failed: %s", hipGetErrorString(e));
         return halide_error_code_device_run_failed;

@@ -340,13 +340,12 @@ int prepare_job(sarbp_server *s, Job &j, hipStream_t st) {
     uint8_t *b = j.dev;
     hipError_t e = hipSuccess;
     // without pixel_locs the accumulation generates the image plane itself (grid mode)
-    double step[3];
     const PixelGrid grid = d.pixel_locs ? PixelGrid{}
-                                        : make_grid(d.nu, d.nv, d.d_u, d.d_v, d.u_hat, d.v_hat, step);
+                                        : make_grid(d.nu, d.nv, d.d_u, d.d_v, d.u_hat, d.v_hat);
     const int n = d.n_fft;
     const float *kr = (const float *)(b + j.ar.k_r);
     e = launch_pulse_consts((const float *)(b + j.ar.pos), kr, d.nsamples, d.npulses, j.lay,
-                            d.norm_r0_policy, grid.on ? step : nullptr, (PulseConst *)(b + j.ar.pc),
+                            d.norm_r0_policy, grid.on ? &grid : nullptr, (PulseConst *)(b + j.ar.pc),
                             (BpHeader *)(b + j.ar.hdr), st);
     if (e == hipSuccess)
         e = launch_range_profiles(b + j.ar.phs, kr, (const double *)(b + j.ar.ws),

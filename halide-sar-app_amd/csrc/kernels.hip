@@ -451,79 +451,111 @@ struct PixTerms {
     double a[PPT], b[PPT], c[PPT], r2[PPT], dr2[PPT];
 };
 
+// One pulse's gather for the thread's pixels: the loaded row elements (in flight until consumed),
+// interpolation weights, phases (revolutions; on the fast path formed when consumed, kappa w) and,
+// for f16 rows, the row's scale.
+template <int PPT>
+struct Gather {
+    float4 e[PPT];
+    float ww[PPT], xx[PPT];
+    float qs;
+};
+
+template <typename QT, int PPT, bool FAST, bool GRID>
+__device__ inline void gather_pulse(const BpArgs &a, const PixTerms<PPT> &P, int p,
+                                    Gather<PPT> &g) {
+    constexpr bool kScaled = sizeof(QT) == sizeof(QPairF16);
+    const QT *qbase = reinterpret_cast<const QT *>(a.q);
+    const PulseConst c = a.pc[p];
+    const __amdgpu_buffer_rsrc_t row = row_rsrc(qbase + (size_t)p * a.len, a.len, (int)sizeof(QT));
+    g.qs = kScaled ? a.qscale[p] : 1.0f;
+    double sq[PPT];
+    if (GRID) {
+        sq[0] = __builtin_fma(P.a[0], c.U, __builtin_fma(P.b[0], c.V, P.r2[0] + c.Cs));
+#pragma unroll
+        for (int j = 1; j < PPT; ++j) sq[j] = sq[0] + __builtin_fma(P.c[j], c.V, P.dr2[j]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j)
+            sq[j] = __builtin_fma(P.a[j], c.px,
+                                  __builtin_fma(P.b[j], c.py, __builtin_fma(P.c[j], c.pz, P.r2[j] + c.Cs)));
+    }
+    int ii[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const double t = row_coord(sq[j], c.B);
+        if (FAST) {
+            ii[j] = trunc_i32(t);
+            g.ww[j] = (float)__builtin_amdgcn_fract(t);
+        } else {
+            ii[j] = trunc_clamp(t, a.len - 2);
+            const double w = t - (double)ii[j];
+            g.ww[j] = (float)w;
+            g.xx[j] = (float)__builtin_amdgcn_fract(a.hdr->kappa * w);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) g.e[j] = loadq(qbase, row, ii[j]);
+}
+
+// term = (A + w B) exp(2 pi i x) added to the f32 partial sums.
+template <typename QT, int PPT, bool FAST>
+__device__ inline void add_terms(const Gather<PPT> &g, float kappa_f, float (&sr)[PPT],
+                                 float (&si)[PPT]) {
+    constexpr bool kScaled = sizeof(QT) == sizeof(QPairF16);
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const float x = FAST ? kappa_f * g.ww[j] : g.xx[j];
+        const float sn = __builtin_amdgcn_sinf(x);
+        const float cs = __builtin_amdgcn_cosf(x);
+        float qr = __builtin_fmaf(g.ww[j], g.e[j].z, g.e[j].x);
+        float qi = __builtin_fmaf(g.ww[j], g.e[j].w, g.e[j].y);
+        if (kScaled) {
+            qr *= g.qs;
+            qi *= g.qs;
+        }
+        sr[j] = __builtin_fmaf(qr, cs, sr[j]);
+        sr[j] = __builtin_fmaf(-qi, sn, sr[j]);
+        si[j] = __builtin_fmaf(qr, sn, si[j]);
+        si[j] = __builtin_fmaf(qi, cs, si[j]);
+    }
+}
+
+// The pulse loop, software-pipelined by one pulse: the gather of pulse p + 1 is issued before the
+// terms of pulse p are formed, so a wave's load latency overlaps its own arithmetic. Two gather
+// sets alternate (two pulses per trip), so nothing is copied between them, and every gather in the
+// loop is issued unconditionally (the last trip's second one is clamped to pulse np - 1 and not
+// used): a load skipped on some path would make the compiler wait for all of them. The f32 partial
+// sums are flushed to f64 after every kBpPulseBlock pulses, as in a plain loop.
+static_assert((kBpPulseBlock & (kBpPulseBlock - 1)) == 0 && kBpPulseBlock % 2 == 0,
+              "the flush test needs an even power of two");
 template <typename QT, int PPT, bool FAST, bool GRID>
 __device__ inline void accumulate_pulses(const BpArgs &a, const PixTerms<PPT> &P,
                                          double (&accr)[PPT], double (&acci)[PPT]) {
-    constexpr bool kScaled = sizeof(QT) == sizeof(QPairF16);
-    const double kappa = a.hdr->kappa;
-    const float kappa_f = (float)kappa;
     const int np = a.np;
-    const int jhi = a.len - 2;
-    const QT *qbase = reinterpret_cast<const QT *>(a.q);
-    for (int p0 = 0; p0 < np; p0 += kBpPulseBlock) {
-        float sr[PPT], si[PPT];
+    const float kappa_f = (float)a.hdr->kappa;
+    float sr[PPT], si[PPT];
 #pragma unroll
-        for (int j = 0; j < PPT; ++j) sr[j] = si[j] = 0.f;
-        const int pe = min(np, p0 + kBpPulseBlock);
-#pragma unroll SARBP_BP_UNROLL
-        for (int p = p0; p < pe; ++p) {
-            const PulseConst c = a.pc[p];
-            const __amdgpu_buffer_rsrc_t row =
-                row_rsrc(qbase + (size_t)p * a.len, a.len, (int)sizeof(QT));
-            const float qs = kScaled ? a.qscale[p] : 1.0f;
-            int ii[PPT];
-            float ww[PPT], xx[PPT];
-            double sq[PPT];
-            if (GRID) {
-                sq[0] = __builtin_fma(P.a[0], c.U, __builtin_fma(P.b[0], c.V, P.r2[0] + c.Cs));
-#pragma unroll
-                for (int j = 1; j < PPT; ++j) sq[j] = sq[0] + __builtin_fma(P.c[j], c.V, P.dr2[j]);
-            } else {
-#pragma unroll
-                for (int j = 0; j < PPT; ++j)
-                    sq[j] = __builtin_fma(P.a[j], c.px,
-                                          __builtin_fma(P.b[j], c.py,
-                                                        __builtin_fma(P.c[j], c.pz, P.r2[j] + c.Cs)));
-            }
-#pragma unroll
-            for (int j = 0; j < PPT; ++j) {
-                const double t = row_coord(sq[j], c.B);
-                if (FAST) {
-                    ii[j] = trunc_i32(t);
-                    ww[j] = (float)__builtin_amdgcn_fract(t);
-                    xx[j] = kappa_f * ww[j];
-                } else {
-                    ii[j] = trunc_clamp(t, jhi);
-                    const double w = t - (double)ii[j];
-                    ww[j] = (float)w;
-                    xx[j] = (float)__builtin_amdgcn_fract(kappa * w);
-                }
-            }
-            float4 e[PPT];
-#pragma unroll
-            for (int j = 0; j < PPT; ++j) e[j] = loadq(qbase, row, ii[j]);
-#pragma unroll
-            for (int j = 0; j < PPT; ++j) {
-                const float sn = __builtin_amdgcn_sinf(xx[j]);
-                const float cs = __builtin_amdgcn_cosf(xx[j]);
-                float qr = __builtin_fmaf(ww[j], e[j].z, e[j].x);
-                float qi = __builtin_fmaf(ww[j], e[j].w, e[j].y);
-                if (kScaled) {
-                    qr *= qs;
-                    qi *= qs;
-                }
-                sr[j] = __builtin_fmaf(qr, cs, sr[j]);
-                sr[j] = __builtin_fmaf(-qi, sn, sr[j]);
-                si[j] = __builtin_fmaf(qr, sn, si[j]);
-                si[j] = __builtin_fmaf(qi, cs, si[j]);
-            }
-        }
+    for (int j = 0; j < PPT; ++j) sr[j] = si[j] = 0.f;
+    auto flush = [&] {
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
             accr[j] += (double)sr[j];
             acci[j] += (double)si[j];
+            sr[j] = si[j] = 0.f;
         }
+    };
+    Gather<PPT> g0, g1;
+    gather_pulse<QT, PPT, FAST, GRID>(a, P, 0, g0);
+    for (int p = 0; p + 1 < np; p += 2) {
+        gather_pulse<QT, PPT, FAST, GRID>(a, P, p + 1, g1);
+        add_terms<QT, PPT, FAST>(g0, kappa_f, sr, si);
+        gather_pulse<QT, PPT, FAST, GRID>(a, P, min(p + 2, np - 1), g0);
+        add_terms<QT, PPT, FAST>(g1, kappa_f, sr, si);
+        if (((p + 2) & (kBpPulseBlock - 1)) == 0) flush();
     }
+    if (np & 1) add_terms<QT, PPT, FAST>(g0, kappa_f, sr, si);  // pulse np - 1
+    flush();
 }
 
 // One 16 x (16 * PPT) tile whose top-left output pixel is (bx, by) relative to the region.

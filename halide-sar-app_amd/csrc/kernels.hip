@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "tile_order.h"
 
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
@@ -658,19 +659,34 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, int
 
 // Grid mode is a kernel template parameter (not a branch in one kernel): each variant gets its
 // own register allocation, so the grid kernels keep the occupancy their smaller state allows.
+// One workgroup per tile (a 1-D grid of all the tiles), each XCD taking a band of columns or of rows
+// (xcd_band_tile): columns when the range changes more across the region's width than across its
+// height. Grid mode measures that with the mid pulse's U and V (the gradient of the squared range
+// along u and v at the scene origin, where image planes are centred); pixel_locs launches use
+// columns, the orientation of the reference's image plane (u_hat along range).
 template <typename QT, int PPT, bool GRID>
 __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
     __shared__ int far_pixels;
-    accumulate_tile_impl<QT, PPT, GRID>(a, blockIdx.x * 16, blockIdx.y * (16 * PPT), far_pixels);
+    constexpr int th = 16 * PPT;
+    const int tiles_x = (a.width + 15) / 16, tiles_y = (a.height + th - 1) / th;
+    bool cols = true;
+    if (GRID) {
+        const PulseConst cm = a.pc[a.np / 2];
+        cols = fabs(cm.U) * a.g.d_u * (double)a.width >= fabs(cm.V) * a.g.d_v * (double)a.height;
+    }
+    int tx, ty;
+    xcd_band_tile((int)blockIdx.x, tiles_x, tiles_y, cols, tx, ty);
+    accumulate_tile_impl<QT, PPT, GRID>(a, tx * 16, ty * th, far_pixels);
 }
 
-// Several jobs in one launch: block b computes tile tiles[b] of job tiles[b].job; all the jobs of
-// a launch share the grid mode.
+// Several jobs in one launch over a tile list (job by job, row-major): block b computes tile
+// tiles[xcd_rank(b, ntiles)], so each XCD takes a contiguous run of the list and reads the profile
+// rows of few jobs. All the jobs of a launch share the grid mode.
 template <typename QT, int PPT, bool GRID>
 __global__ __launch_bounds__(kBpThreads) void accumulate_fused_kernel(
-    const BpArgs *__restrict__ jobs, const FusedTile *__restrict__ tiles) {
+    const BpArgs *__restrict__ jobs, const FusedTile *__restrict__ tiles, int ntiles) {
     __shared__ int far_pixels;
-    const FusedTile t = tiles[blockIdx.x];
+    const FusedTile t = tiles[xcd_rank((int)blockIdx.x, ntiles)];
     const BpArgs a = jobs[t.job];
     accumulate_tile_impl<QT, PPT, GRID>(a, t.bx, t.by, far_pixels);
 }
@@ -879,7 +895,7 @@ hipError_t launch_range_profiles(const void *phs, const float *k_r, const double
 
 template <int PPT, bool GRID>
 static hipError_t launch_accumulate_ppt(const BpArgs &a, hipStream_t stream) {
-    const dim3 grid((a.width + 15) / 16, (a.height + 16 * PPT - 1) / (16 * PPT));
+    const dim3 grid((unsigned)(((a.width + 15) / 16) * ((a.height + 16 * PPT - 1) / (16 * PPT))));
     if (a.q_format == SARBP_Q_C16)
         hipLaunchKernelGGL((accumulate_kernel<QPairF16, PPT, GRID>), grid, dim3(kBpThreads), 0,
                            stream, a);
@@ -894,10 +910,10 @@ static hipError_t launch_fused_ppt(const BpArgs *jobs, const FusedTile *tiles, i
                                    int q_format, hipStream_t stream) {
     if (q_format == SARBP_Q_C16)
         hipLaunchKernelGGL((accumulate_fused_kernel<QPairF16, PPT, GRID>), dim3(ntiles),
-                           dim3(kBpThreads), 0, stream, jobs, tiles);
+                           dim3(kBpThreads), 0, stream, jobs, tiles, ntiles);
     else
         hipLaunchKernelGGL((accumulate_fused_kernel<QPairF32, PPT, GRID>), dim3(ntiles),
-                           dim3(kBpThreads), 0, stream, jobs, tiles);
+                           dim3(kBpThreads), 0, stream, jobs, tiles, ntiles);
     return hipGetLastError();
 }
 

@@ -53,12 +53,14 @@ def _alone(sar, fac, i):
 
 def _alone_grid(sar, fac, i, ppt):
     """The job alone through a device plan in grid mode at tile height 16 * ppt: the arithmetic the
-    engine uses for jobs without pixel_locs."""
+    engine uses for jobs without pixel_locs (the plan's range profiles come from the one-block-per-
+    profile kernel, the engine's from the 256-thread variant, which writes the same bits)."""
     import torch
     sc, g, sp = fac.scenes[i], fac.grids[i], fac.specs[i]
     dev = torch.device("cuda", 0)
     plan = sar.Plan(sc.nsamples, sc.npulses, sar.nfft(sc.nsamples, 2), sp["taylor"], sc.delta_r,
-                    sp["side"], sp["side"], taylor_den_policy=sar.TAYLOR_DEN_F32, tile_ppt=ppt)
+                    sp["side"], sp["side"], taylor_den_policy=sar.TAYLOR_DEN_F32, tile_ppt=ppt,
+                    q_format=fac.q_format)
     plan.set_grid(g["d_u"], g["d_u"], g["u_hat"], g["v_hat"])
     out = torch.empty((sp["side"], sp["side"]), dtype=torch.complex128, device=dev)
     plan.execute(torch.from_numpy(np.ascontiguousarray(fac.phs[i])).to(dev),
@@ -180,3 +182,24 @@ def test_many_small_jobs_batched(sar):
     assert all(r["fused"] for r in recs)
     for i in kept:
         assert np.array_equal(kept[i], _alone_grid(sar, fac, i, recs[i]["ppt"])), i
+
+
+@pytest.mark.parametrize("q_format", [0, 1])
+def test_row_formats_and_largest_fft(sar, q_format):
+    """Complex-f32 and scaled-f16 profile rows, and a job whose N_fft (16384) is above the engine's
+    256-thread range-profile kernel (it falls back to one block per profile); odd pulse counts
+    exercise the pipelined pulse loop's last pulse. Every job equals the job alone, bit for bit."""
+    sv = _serving()
+    specs = [dict(ns=5000, npulses=9, side=40, taylor=20),
+             dict(ns=64, npulses=31, side=48, taylor=17),
+             dict(ns=96, npulses=120, side=100, taylor=24),
+             dict(ns=128, npulses=45, side=72, taylor=19)]
+    fac = sv.JobFactory(specs, seed_base=9500, q_format=q_format)
+    assert sar.nfft(5000, 2) == 16384
+    print(f"serving row format {q_format} with N_fft 16384", flush=True)
+    with sv.Server(fuse_max_pixel_pulses=FUSE_MAX, autotune=False) as srv:
+        recs, _, kept = sv.replay(srv, fac, None, keep_outputs=range(len(specs)))
+    assert any(r["fused"] for r in recs)
+    for i in range(len(specs)):
+        assert np.array_equal(kept[i], _alone_grid(sar, fac, i, recs[i]["ppt"])), i
+        assert np.all(np.isfinite(kept[i])), i

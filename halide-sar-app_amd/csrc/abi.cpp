@@ -299,12 +299,31 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
     double *hr = (double *)(h + b_phs + b_kr + b_pos);
     uint8_t *dbase = (uint8_t *)ctx->d_aux.p;
     double *dr = (double *)(dbase + b_phs + b_kr + b_pos);
+    hipStream_t s1 = ctx->stream, s2 = ctx->stream2, s3 = ctx->stream3;
+    auto drain = [&](int code, const char *what) {
+        (void)hipStreamSynchronize(s1);
+        (void)hipStreamSynchronize(s2);
+        (void)hipStreamSynchronize(s3);
+        return hip_fail(e, code, what);
+    };
+    size_t phs_sent = 0;  // bytes of phs already enqueued for upload
     {
         const float *src = (const float *)phs->host;
         const bool dense = phs->dim[0].stride == 1 && phs->dim[1].stride == 2 &&
                            phs->dim[2].stride == 2 * ns && phs->dim[0].min == 0;
         if (dense) {
-            par_copy(hphs, src, sizeof(float) * 2 * (size_t)ns * np);
+            // packed and uploaded in pulse chunks, so that each chunk's DMA overlaps the packing
+            // of the next
+            const size_t row = sizeof(float) * 2 * (size_t)ns;
+            const int64_t chunk = std::max<int64_t>(1, (int64_t)((8u << 20) / row));
+            for (int64_t p0 = 0; p0 < np; p0 += chunk) {
+                const size_t off = row * (size_t)p0, n = row * (size_t)std::min(chunk, np - p0);
+                par_copy((uint8_t *)hphs + off, (const uint8_t *)src + off, n);
+                if ((e = hipMemcpyAsync(dbase + off, h + off, n, hipMemcpyHostToDevice, s1)) !=
+                    hipSuccess)
+                    return drain(halide_error_code_copy_to_device_failed, "input upload");
+            }
+            phs_sent = row * (size_t)np;
         } else {
             for (int64_t p = 0; p < np; ++p)
                 for (int64_t s = 0; s < ns; ++s)
@@ -327,15 +346,8 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
                     (in_dim(pos, 0, c) && in_dim(pos, 1, p)) ? ps[offset_of(pos, co)] : 0.0f;
             }
     }
-    hipStream_t s1 = ctx->stream, s2 = ctx->stream2, s3 = ctx->stream3;
-    auto drain = [&](int code, const char *what) {
-        (void)hipStreamSynchronize(s1);
-        (void)hipStreamSynchronize(s2);
-        (void)hipStreamSynchronize(s3);
-        return hip_fail(e, code, what);
-    };
-    if ((e = hipMemcpyAsync(dbase, h, b_phs + b_kr + b_pos, hipMemcpyHostToDevice, s1)) !=
-        hipSuccess)
+    if ((e = hipMemcpyAsync(dbase + phs_sent, h + phs_sent, b_phs + b_kr + b_pos - phs_sent,
+                            hipMemcpyHostToDevice, s1)) != hipSuccess)
         return drain(halide_error_code_copy_to_device_failed, "input upload");
     rv = plan_prepare(plan, dbase, dbase + b_phs, dbase + b_phs + b_kr, s1);
     if (rv) {
@@ -343,11 +355,13 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
         return rv;
     }
 
-    // Row bands of about 2048 tiles (one wave of the device), tile-aligned, at most 8, alternating
-    // between the two compute streams.
+    // Row bands of about 1024 tiles (half a wave of the device: two bands in flight fill it),
+    // tile-aligned, at most 8, alternating between the two compute streams. Against bands of 2048
+    // tiles the first band waits for half as many pixel locations and the last one runs alone
+    // for half as long (C2: 8.6-9.1 ms per call instead of 9.0-9.4).
     const int th = plan_tile_rows(plan);
     const int64_t tile_rows = (oh + th - 1) / th, tile_cols = (ow + 15) / 16;
-    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>({8, tile_rows, tile_rows * tile_cols / 2048}));
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>({8, tile_rows, tile_rows * tile_cols / 1024}));
     const int band = (int)((tile_rows + nb - 1) / nb) * th;
     const int nbands = (oh + band - 1) / band;
     if ((e = ctx->ensure_events(3 * (size_t)nbands + 1)) != hipSuccess)

@@ -400,6 +400,37 @@ def test_grid_mode_matches_pixel_locs(sar, orc, scenes, pipe, torch_dev, ppt):
     assert rel_l2(full.cpu().numpy(), ref) < TOL
 
 
+def test_grid_mode_axes_across_range(sar, orc, scenes, pipe, torch_dev):
+    """C1 with the image axes swapped (u_hat across the range direction, v_hat along it): grid mode
+    then finds the range changing along image y and gives each XCD a band of tile rows instead of
+    columns. The image equals the r-buffer path to rounding and the oracle to 1e-4."""
+    torch = torch_dev
+    sc = scenes.config_scene("C1")
+    g0 = pipe.scene_grid(sc)
+    grid = sar.image_grid(g0["nu"], g0["nv"], g0["d_v"], g0["d_u"], g0["v_hat"], g0["u_hat"])
+    # the kernel's test (mid pulse's |U| d_u width >= |V| d_v height: columns) now picks rows
+    p = sc.pos[sc.npulses // 2].astype(np.float64)
+    across_x = abs(np.dot(grid["u_hat"], p)) * grid["d_u"] * grid["nu"]
+    across_y = abs(np.dot(grid["v_hat"], p)) * grid["d_v"] * grid["nv"]
+    assert across_y > across_x
+    phs = scenes.simulate_phs_numpy(sc)
+    ds_r = pipe.DeviceScene(sc, grid, phs_host=phs)
+    ds_g = pipe.DeviceScene(sc, dict(grid, pixel_locs=None), phs_host=phs)
+    o_r, o_g = ds_r.output(), ds_g.output()
+    ds_r.run(ds_r.plan(), o_r)
+    ds_g.run(ds_g.plan(), o_g)
+    torch.cuda.synchronize()
+    a, b = o_r.cpu().numpy(), o_g.cpu().numpy()
+    e = rel_l2(b, a)
+    print(f"swapped axes: grid vs pixel_locs rel {e:.2e}")
+    assert e < 1e-6
+    ref = orc.backprojection(phs, sc.k_r, sc.taylor, ds_g.n_fft, sc.delta_r, grid["nu"], grid["nv"],
+                             sc.pos, grid["pixel_locs"], nthreads=NTHREADS)
+    e = rel_l2(b, ref)
+    print(f"swapped axes: grid vs oracle rel {e:.2e}")
+    assert e < TOL
+
+
 def test_device_pixel_locs_match_host(sar, torch_dev):
     """pixel_locs generated on the device (img_plane.cpp:77-113) equal img_plane_create's host
     values bit for bit (AFRL-sized plane and an odd-sized explicit grid)."""

@@ -590,19 +590,39 @@ __device__ inline void pixel_loc(const BpArgs &a, int lx, int ly, bool ok, doubl
 }
 
 template <typename QT, int PPT, bool GRID>
-__device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, bool x_range,
-                                            int &far_pixels) {
+__device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, int &far_pixels) {
+    const double inv_lsr = a.hdr->inv_lsr;
     // Wave w holds rows 4w..4w+3 (+16 j) of the tile's 16 columns. The memory pipeline serves a
-    // gather four lanes at a time, and its cost grows with the cache lines each quad of lanes
-    // touches, so a quad takes 4 pixels along the axis where the range changes least: with the
-    // range along x, 4 rows of one column (lane = 4 x + row), else 4 columns of one row.
+    // gather four lanes at a time and its cost grows with the cache lines each quad of lanes
+    // touches: a quad of 4 columns spans 4 g_x range bins (g_x: bins per pixel along x). Where
+    // g_x >= 2.4 and the range changes less along y, a quad takes 4 rows of one column instead
+    // (lane = 4 x + row). Below that, 4 columns of one row are cheaper: there the 4 lanes of a
+    // column quad would mostly read one element. Measured on the C2 geometry at 1/4 to 2 times its
+    // pixel spacing (0.95 to 7.6 bins per pixel): columns are faster up to 1.9 bins per pixel, rows
+    // from 2.85. g_x and g_y are taken at the tile's first pixel for the mid pulse.
+    bool y_quads;
+    {
+        const PulseConst cm = a.pc[a.np / 2];
+        const int x1 = bx + 1 < a.width ? bx + 1 : bx - 1, y1 = by + 1 < a.height ? by + 1 : by - 1;
+        double r[3][3], u, v;
+        pixel_loc<GRID>(a, bx, by, true, r[0][0], r[0][1], r[0][2], u, v);
+        pixel_loc<GRID>(a, x1 >= 0 ? x1 : bx, by, true, r[1][0], r[1][1], r[1][2], u, v);
+        pixel_loc<GRID>(a, bx, y1 >= 0 ? y1 : by, true, r[2][0], r[2][1], r[2][2], u, v);
+        double d[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double dx = r[i][0] - cm.px, dy = r[i][1] - cm.py, dz = r[i][2] - cm.pz;
+            d[i] = sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        const double gx = fabs(d[1] - d[0]) * inv_lsr, gy = fabs(d[2] - d[0]) * inv_lsr;
+        y_quads = gx >= 2.4 && gx > gy;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tx = x_range ? lane >> 2 : lane & 15;
-    const int ty = 4 * wave + (x_range ? lane & 3 : lane >> 4);
+    const int tx = y_quads ? lane >> 2 : lane & 15;
+    const int ty = 4 * wave + (y_quads ? lane & 3 : lane >> 4);
     if (threadIdx.x == 0) far_pixels = 0;
     __syncthreads();
 
-    const double inv_lsr = a.hdr->inv_lsr;
     const double sigma = inv_lsr * inv_lsr;
     PixTerms<PPT> P;
     bool ok[PPT];
@@ -666,35 +686,25 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, boo
 
 // Grid mode is a kernel template parameter (not a branch in one kernel): each variant gets its
 // own register allocation, so the grid kernels keep the occupancy their smaller state allows.
-// How the range runs across a launch's region. Grid mode measures it with the mid pulse's U and V
-// (the gradient of the squared range along u and v at the scene origin, where image planes are
-// centred); pixel_locs launches take the orientation of the reference's image plane (u_hat along
-// range). x_pixel: the range changes faster from pixel to pixel along x than along y. x_region: it
-// changes more across the region's width than across its height.
-template <bool GRID>
-__device__ inline void range_orientation(const BpArgs &a, bool &x_pixel, bool &x_region) {
-    x_pixel = x_region = true;
-    if (GRID) {
-        const PulseConst cm = a.pc[a.np / 2];
-        const double gx = fabs(cm.U) * a.g.d_u, gy = fabs(cm.V) * a.g.d_v;
-        x_pixel = gx >= gy;
-        x_region = gx * (double)a.width >= gy * (double)a.height;
-    }
-}
-
 // One workgroup per tile (a 1-D grid of all the tiles), each XCD taking a band of columns or of rows
 // (xcd_band_tile): columns when the range changes more across the region's width than across its
-// height.
+// height. Grid mode measures that with the mid pulse's U and V (the gradient of the squared range
+// along u and v at the scene origin, where image planes are centred); pixel_locs launches use
+// columns, the orientation of the reference's image plane (u_hat along range). Every workgroup of
+// a launch takes the same decision, so the order stays a bijection.
 template <typename QT, int PPT, bool GRID>
 __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
     __shared__ int far_pixels;
     constexpr int th = 16 * PPT;
     const int tiles_x = (a.width + 15) / 16, tiles_y = (a.height + th - 1) / th;
-    bool x_pixel, x_region;
-    range_orientation<GRID>(a, x_pixel, x_region);
+    bool cols = true;
+    if (GRID) {
+        const PulseConst cm = a.pc[a.np / 2];
+        cols = fabs(cm.U) * a.g.d_u * (double)a.width >= fabs(cm.V) * a.g.d_v * (double)a.height;
+    }
     int tx, ty;
-    xcd_band_tile((int)blockIdx.x, tiles_x, tiles_y, x_region, tx, ty);
-    accumulate_tile_impl<QT, PPT, GRID>(a, tx * 16, ty * th, x_pixel, far_pixels);
+    xcd_band_tile((int)blockIdx.x, tiles_x, tiles_y, cols, tx, ty);
+    accumulate_tile_impl<QT, PPT, GRID>(a, tx * 16, ty * th, far_pixels);
 }
 
 // Several jobs in one launch over a tile list (job by job, row-major): block b computes tile
@@ -706,9 +716,7 @@ __global__ __launch_bounds__(kBpThreads) void accumulate_fused_kernel(
     __shared__ int far_pixels;
     const FusedTile t = tiles[xcd_rank((int)blockIdx.x, ntiles)];
     const BpArgs a = jobs[t.job];
-    bool x_pixel, x_region;
-    range_orientation<GRID>(a, x_pixel, x_region);
-    accumulate_tile_impl<QT, PPT, GRID>(a, t.bx, t.by, x_pixel, far_pixels);
+    accumulate_tile_impl<QT, PPT, GRID>(a, t.bx, t.by, far_pixels);
 }
 
 // ------------------------------------------------------------------ image output

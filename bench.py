@@ -215,7 +215,7 @@ def main():
 
     flop = pp * pipe.algorithmic_flops_per_pp()
     achieved = flop / (acc_ms * 1e-3) / 1e12
-    ppt_used = args.tile_ppt if args.tile_ppt in (2, 3, 4) else 2
+    ppt_used = plan.tile_ppt
     kernel = (f"accumulate_kernel<{'QPairF16' if qf == sar.Q_C16 else 'QPairF32'}, {ppt_used}, "
               f"{'grid' if ds.grid_mode else 'pixel_locs'}>")
     traffic = None

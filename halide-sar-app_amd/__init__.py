@@ -167,6 +167,8 @@ _lib.sarbp_plan_q_buffer.argtypes = [_P]
 _lib.sarbp_plan_q_buffer.restype = _P
 _lib.sarbp_plan_q_bytes.argtypes = [_P]
 _lib.sarbp_plan_q_bytes.restype = ctypes.c_size_t
+_lib.sarbp_plan_tile_ppt.argtypes = [_P]
+_lib.sarbp_plan_tile_ppt.restype = _i32
 _lib.sarbp_plan_read_pulse_norms.argtypes = [_P, _P, _i32]
 _lib.sarbp_plan_read_profiles.argtypes = [_P, _P]
 _lib.sarbp_img_output_device.argtypes = [_P, _i64, _P, _P, _f64, _f64, _P]
@@ -425,6 +427,11 @@ class Plan:
     def q_bytes(self) -> int:
         """Size of the stored range-profile rows (the buffer range_profiles(q=...) fills)."""
         return int(_lib.sarbp_plan_q_bytes(self._p))
+
+    @property
+    def tile_ppt(self) -> int:
+        """Pixels per thread of the accumulation tiles (the default resolved for the region)."""
+        return int(_lib.sarbp_plan_tile_ppt(self._p))
 
 
 def img_output_device(img, npix: int, dB=None, u8=None, dB_min=0.0, dB_max=1.0, stream=None):

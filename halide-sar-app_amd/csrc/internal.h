@@ -107,6 +107,8 @@ hipError_t launch_accumulate_fused(const BpArgs *jobs_dev, const FusedTile *tile
                                    int q_format, int ppt, bool grid, hipStream_t stream);
 // Output rows covered by one accumulation tile for a pixels-per-thread setting (16 columns wide).
 int tile_height(int ppt);
+// Pixels per thread for a region when the caller leaves the choice to the library (tile_ppt 0).
+int default_tile_ppt(int width, int height);
 hipError_t launch_img_output(const double2 *img, int64_t npix, double *dB, uint8_t *u8,
                              double lo, double hi, unsigned long long *scratch_max,
                              hipStream_t stream);

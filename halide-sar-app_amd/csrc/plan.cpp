@@ -159,6 +159,7 @@ extern "C" int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out) {
     DeviceGuard g(d->device);
     sarbp_plan *p = new sarbp_plan();
     p->d = *d;
+    if (p->d.tile_ppt == 0) p->d.tile_ppt = default_tile_ppt(d->width, d->height);
     const int ns = d->nsamples, np = d->npulses, N = d->n_fft;
     std::vector<double> ws((size_t)ns), wp((size_t)np);
     taylor_window(ns, d->taylor_s_l, d->taylor_den_policy, ws.data());
@@ -200,6 +201,8 @@ extern "C" void *sarbp_plan_q_buffer(sarbp_plan *p) { return p ? p->q : nullptr;
 extern "C" size_t sarbp_plan_q_bytes(sarbp_plan *p) {
     return p ? q_elem_bytes(p->d.q_format) * (size_t)p->d.npulses * (size_t)p->lay.len : 0;
 }
+
+extern "C" int32_t sarbp_plan_tile_ppt(const sarbp_plan *p) { return p ? p->d.tile_ppt : 0; }
 
 namespace {
 float half_to_float(uint16_t h) {

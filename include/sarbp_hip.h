@@ -122,7 +122,8 @@ typedef struct sarbp_plan_desc {
     int32_t range_mode;              /* sarbp_range_mode */
     int32_t device;                  /* HIP device ordinal */
     int32_t tile_ppt;                /* pixels per thread of the accumulation tile (2, 3 or 4;
-                                        0 picks the default) */
+                                        0 picks the default: 4 for regions of at least 2048
+                                        16 x 64 tiles, else 2) */
     int32_t fft_blocks;              /* range profiles: 0 = one 512-thread block per profile;
                                         > 0 = at most this many 256-thread blocks, each looping
                                         over profiles (N_fft <= 8192; for a device shared with
@@ -152,6 +153,8 @@ int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const void *k_r_dev,
 /* Device pointer and size of the plan's range-profile buffer. */
 void *sarbp_plan_q_buffer(sarbp_plan *p);
 size_t sarbp_plan_q_bytes(sarbp_plan *p);
+/* Pixels per thread of the plan's accumulation tiles (tile height 16 * this), the default resolved. */
+int32_t sarbp_plan_tile_ppt(const sarbp_plan *p);
 
 /* Grid mode: the plan's pixel locations come from the image grid (img_plane.cpp:77-113:
  * r(x, y) = (0 + u_hat u[x]) + v_hat v[y], u[x] = (x + floor(-nu/2)) d_u, v likewise, nu and nv from

@@ -96,11 +96,12 @@ def test_partial_region_and_strided_output(sar, torch_dev):
     args = (g["phs"], g["k_r"], int(g["taylor"]), int(g["n_fft"]), float(g["delta_r"]))
     full = run_abi(sar, *args, nu, nv, g["pos"], g["pixel_locs"])
     u, v = np.zeros(nu), np.zeros(nv)
-    # a 7 x 5 window at (x, y) = (3, 4)
+    # a 7 x 5 window at (x, y) = (3, 4): its tiles start at the window's corner, so the ranges are
+    # expanded about other centres than in the full image (agreement to rounding, not bits)
     sub = np.zeros((5, 7), np.complex128)
     ob = sar.Buffer(sub, halide_dims=[(0, 2, 1), (3, 7, 2), (4, 5, 14)])
     assert sar.backprojection(*args, u, v, g["pos"], g["pixel_locs"], ob) == 0, sar.last_error()
-    assert np.array_equal(sub, full[4:9, 3:10])
+    assert rel_l2(sub, full[4:9, 3:10]) < 1e-6
     # strided (transposed) output and only the real part
     big = np.zeros((nu, nv, 2))
     ob = sar.Buffer(big, halide_dims=[(0, 1, 1), (0, nu, 2 * nv), (0, nv, 2)])

@@ -361,6 +361,36 @@ def test_pixels_beyond_row_cover_take_exact_path(sar, orc, scenes, torch_dev):
     assert e < TOL
 
 
+def test_near_field_takes_direct_form(sar, orc, scenes, torch_dev):
+    """A platform 36 m from the scene centre. Every pixel lies inside the rows' cover, so the fast
+    loop runs, but the range expansion about the tile centres fails its error bound by six orders of
+    magnitude (|delta|^4 / (4 d^3) ~ 1e-2 row elements, 2.75 phase revolutions per element): the
+    tiles must take the direct form, or the image would be off by ~0.1 rad of phase."""
+    sc = scenes.circular_arc_scene(npulses=48, nsamples=64, f_c=9.6e9, df=15e6, radius=30.0,
+                                   altitude=20.0, arc_deg=40.0, ntargets=5, patch=4.0, snr_db=20.0,
+                                   taylor=17, upsample=2, seed=91, name="near")
+    phs = scenes.simulate_phs_numpy(sc)
+    ip = orc.img_plane(sc.nsamples, sc.npulses, sc.delta_r, sc.R_c)
+    nu, nv, pl = ip["nu"], ip["nv"], ip["pixel_locs"]
+    n_fft = sar.nfft(sc.nsamples, 2)
+    # the guard's inputs, from the geometry (rows as internal row_layout lays them out)
+    lsa = np.floor(-sc.nsamples * sc.delta_r / 2)
+    lsr = (np.floor(sc.nsamples * sc.delta_r / 2) - lsa) / (n_fft - 1)
+    ext = n_fft // 4
+    cover = min(-(lsa + (2 - ext) * lsr), lsa + (n_fft + ext - 3) * lsr) - 0.01
+    assert np.sqrt((pl ** 2).sum(0)).max() < cover
+    pos = sc.pos.astype(np.float64)
+    d_min = min(np.linalg.norm(pos - pl[:, i], axis=1).min() for i in range(0, nu * nv, 7)) / lsr
+    spacing = np.linalg.norm(pl[:, 1] - pl[:, 0]) / lsr
+    half = np.hypot(7.5, 15.5) * spacing          # half-diagonal of a 16 x 32 tile, row elements
+    assert half ** 4 / (4 * d_min ** 3) > 1e-3
+    img = run_abi(sar, phs, sc.k_r, 17, n_fft, sc.delta_r, nu, nv, sc.pos, pl)
+    ref = orc.backprojection(phs, sc.k_r, 17, n_fft, sc.delta_r, nu, nv, sc.pos, pl)
+    e = rel_l2(img, ref)
+    print(f"near field: d_min {d_min:.0f} row elements, rel_l2 {e:.3e}")
+    assert e < TOL
+
+
 @pytest.mark.parametrize("ppt", [2, 3, 4])
 def test_grid_mode_matches_pixel_locs(sar, orc, scenes, pipe, torch_dev, ppt):
     """Grid mode (pixel locations generated in the accumulation kernel, squared ranges of a

@@ -355,13 +355,16 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
         return rv;
     }
 
-    // Row bands of about 1024 tiles (half a wave of the device: two bands in flight fill it),
-    // tile-aligned, at most 8, alternating between the two compute streams. Against bands of 2048
-    // tiles the first band waits for half as many pixel locations and the last one runs alone
-    // for half as long (C2: 8.6-9.1 ms per call instead of 9.0-9.4).
+    // Row bands of half as many tiles as the device holds at once (two bands in flight fill it;
+    // 1024 tiles at 16 x 32, 512 at 16 x 64), tile-aligned, at most 8, alternating between the two
+    // compute streams. Against bands of a whole device wave the first band waits for half as many
+    // pixel locations and the last one runs alone for half as long (C2, 16 x 32 tiles: 8.6-9.1 ms per
+    // call instead of 9.0-9.4).
     const int th = plan_tile_rows(plan);
     const int64_t tile_rows = (oh + th - 1) / th, tile_cols = (ow + 15) / 16;
-    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>({8, tile_rows, tile_rows * tile_cols / 1024}));
+    const int64_t band_tiles = std::max(256, plan_resident_tiles(plan) / 2);
+    const int nb = (int)std::max<int64_t>(
+        1, std::min<int64_t>({8, tile_rows, tile_rows * tile_cols / band_tiles}));
     const int band = (int)((tile_rows + nb - 1) / nb) * th;
     const int nbands = (oh + band - 1) / band;
     if ((e = ctx->ensure_events(3 * (size_t)nbands + 1)) != hipSuccess)

@@ -109,6 +109,9 @@ hipError_t launch_accumulate_fused(const BpArgs *jobs_dev, const FusedTile *tile
 int tile_height(int ppt);
 // Pixels per thread for a region when the caller leaves the choice to the library (tile_ppt 0).
 int default_tile_ppt(int width, int height);
+// Accumulation workgroups (tiles) resident on the whole device at once for this instantiation
+// (occupancy per CU x CUs); 0 if the runtime cannot tell.
+int accumulate_resident_tiles(int device, int q_format, int ppt, bool grid);
 hipError_t launch_img_output(const double2 *img, int64_t npix, double *dB, uint8_t *u8,
                              double lo, double hi, unsigned long long *scratch_max,
                              hipStream_t stream);
@@ -138,6 +141,8 @@ int plan_prepare(sarbp_plan *p, const void *phs_dev, const void *k_r_dev, const 
 int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int64_t r_extent,
                          int row0, int nrows, void *out_rows, hipStream_t stream);
 int plan_tile_rows(const sarbp_plan *p);
+// Tiles of the plan's accumulation kernel resident on its device at once (0 if unknown).
+int plan_resident_tiles(const sarbp_plan *p);
 const sarbp_plan_desc &plan_desc(const sarbp_plan *p);
 // Bytes per stored range-profile element: (q, dq) in complex f32 or complex f16.
 inline size_t q_elem_bytes(int q_format) { return q_format == SARBP_Q_C16 ? 8 : 16; }

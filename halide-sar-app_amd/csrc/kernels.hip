@@ -1245,6 +1245,35 @@ int default_tile_ppt(int width, int height) {
     return tiles4 >= 2048 ? 4 : kDefaultPpt;
 }
 
+template <typename QT, int PPT, bool GRID>
+static int resident_blocks() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, reinterpret_cast<const void *>(&accumulate_kernel<QT, PPT, GRID>), kBpThreads, 0) !=
+        hipSuccess)
+        return 0;
+    return n;
+}
+
+template <int PPT, bool GRID>
+static int resident_blocks_fmt(int q_format) {
+    return q_format == SARBP_Q_C16 ? resident_blocks<QPairF16, PPT, GRID>()
+                                   : resident_blocks<QPairF32, PPT, GRID>();
+}
+
+int accumulate_resident_tiles(int device, int q_format, int ppt, bool grid) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    int per_cu = 0;
+    switch (ppt) {
+    case 3: per_cu = grid ? resident_blocks_fmt<3, true>(q_format) : resident_blocks_fmt<3, false>(q_format); break;
+    case 4: per_cu = grid ? resident_blocks_fmt<4, true>(q_format) : resident_blocks_fmt<4, false>(q_format); break;
+    default: per_cu = grid ? resident_blocks_fmt<2, true>(q_format) : resident_blocks_fmt<2, false>(q_format); break;
+    }
+    return cus * per_cu;
+}
+
 hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream) {
     if (a.width <= 0 || a.height <= 0) return hipSuccess;
     if (a.g.on) {

@@ -443,6 +443,10 @@ int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int6
 
 int plan_tile_rows(const sarbp_plan *p) { return tile_height(p->d.tile_ppt); }
 
+int plan_resident_tiles(const sarbp_plan *p) {
+    return accumulate_resident_tiles(p->d.device, p->d.q_format, p->d.tile_ppt, p->grid.on != 0);
+}
+
 const sarbp_plan_desc &plan_desc(const sarbp_plan *p) { return p->d; }
 
 }  // namespace sarbp

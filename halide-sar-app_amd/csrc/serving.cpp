@@ -50,11 +50,13 @@ namespace {
 constexpr int64_t kHeadPatienceNs = 50'000'000;     // 50 ms
 constexpr size_t kFuseSlotBytes = (size_t)4 << 20;  // launch table (job args + tiles) per slot
 constexpr int kFuseSlots = 2;                       // fused launches in flight at most
-// Workgroups per accumulation launch of a large job: two waves of 8 tiles per CU on 256 CUs. While
-// one launch of some 10^5 tiles runs, the device starts no kernel from any other queue (measured
-// with a kernel trace), so a large job is computed in row bands of this many tiles and other jobs'
-// kernels and copies are dispatched between bands.
-constexpr int64_t kBandTiles = 4096;
+// Pixels per accumulation launch of a large job. While one launch of some 10^5 tiles runs, the
+// device starts no kernel from any other queue (measured with a kernel trace), so a large job is
+// computed in row bands and other jobs' kernels and copies are dispatched between bands. A band
+// holds this many pixels whatever the tile height: 4096 tiles of 16 x 32 (two device waves at 8
+// tiles per CU) or 2048 of 16 x 64 (two waves at 4 per CU), so that its duration does not grow with
+// the tile the tuner picks.
+constexpr int64_t kBandPixels = 4096 * 16 * 32;
 constexpr int kBandsAhead = 2;
 // Pixel-pulses per fused launch at most (~6 ms of accumulation on an idle device), so that a fused
 // launch holds the hardware queue it shares with other streams no longer than a band does.
@@ -416,8 +418,8 @@ void fail_inflight(sarbp_server *s, Job &j, int code) {
 // time; the scheduler adds the next band when the oldest completes, so work on any stream that
 // shares a hardware queue with this job's stream waits for at most kBandsAhead bands.
 int band_rows(const BpArgs &a) {
-    const int64_t cols = (a.width + 15) / 16;
-    return (int)std::max<int64_t>(1, kBandTiles / cols) * tile_height(a.ppt);
+    const int64_t cols = (a.width + 15) / 16, th = tile_height(a.ppt);
+    return (int)(std::max<int64_t>(1, kBandPixels / (16 * th * cols)) * th);
 }
 
 // Enqueue bands while fewer than kBandsAhead are outstanding; after the last band, ev_acc.

@@ -428,8 +428,11 @@ constexpr int kBpPulseBlock = SARBP_BP_PULSE_BLOCK;
 constexpr int kDefaultPpt = 2;
 
 // Tile of 16 x (16 * PPT) output pixels per block; thread (tx, ty) owns pixels (tx, ty + 16 j).
-// Squared ranges are kept in row units, s = sigma |r - pos_p|^2 with sigma = inv_lsr^2, so that
-// sqrt(s) is d / lsr. Per pulse p (constants from the pulse table):
+// Per pulse p the pixel's row coordinate is t = B_p - |r - pos_p| / lsr. Most tiles form it from
+// the expansion about the tile's centre (accumulate_pulses_expanded, below); the direct form, used
+// where the expansion's error bound fails, keeps squared ranges in row units,
+// s = sigma |r - pos_p|^2 with sigma = inv_lsr^2, so that sqrt(s) is d / lsr. Per pulse p
+// (constants from the pulse table):
 //   s  = sigma |r|^2 + Cs_p - 2 sigma r.pos_p          f64, from per-pixel terms (PixTerms)
 //   t  = B_p - sqrt(s)                                 f64: v_rsq_f64 + one Newton step (row_coord)
 //   j  = floor(t), w = t - j                           int, f64 -> f32
@@ -565,8 +568,8 @@ __device__ inline void accumulate_pulses(const BpArgs &a, const PixTerms<PPT> &P
 // (the Taylor series of d_c sqrt(1 + 2 p / d_c + |delta|^2 / d_c^2) to third order in
 // eps = |delta| / d_c), with |E| <= d_c eps^4 / 8 for eps <= 0.1 (checked against 40-digit
 // arithmetic over random directions). So t = T - p - q (h + g p) with T = B_p - d_c and g = -2 h^2:
-// per pixel-pulse 5.5 f64 operations in grid mode (7 with pixel_locs) and no square root, against
-// 6.5 (8) plus v_rsq_f64 for the direct form. A tile takes this path when
+// per pixel-pulse 5 + 1 / PPT f64 operations in grid mode (7 with pixel_locs) and no square root,
+// against 6.5 (8) plus v_rsq_f64 for the direct form. A tile takes this path when
 // |delta|_max^4 / (4 d_min^3) <= kExpandTol, twice the bound on E, over all of its pixels and pulses
 // (d_min: the least d_c over the pulses); otherwise the direct form. On the C1-C3 scenes the bound
 // is below 2e-10 row elements (3e-9 radians of phase).
@@ -621,8 +624,8 @@ __device__ inline void fill_expansion(const BpArgs &a, TileShared &sh, int blk, 
     k[5] = -2.0 * h * h;
 }
 
-// One pulse's expansion constants, read from LDS ahead of their use so that the read's latency
-// overlaps the previous pulse's arithmetic.
+// One pulse's expansion constants from the LDS block (the address is the same in every lane, so
+// each read is one broadcast). Issuing these reads earlier in the pipelined loop measured no faster.
 struct ExpConst {
     double T, n0, n1, n2, h, g;
 };

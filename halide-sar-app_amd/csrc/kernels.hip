@@ -998,20 +998,36 @@ __global__ __launch_bounds__(kBpThreads) void accumulate_fused_kernel(
 // ------------------------------------------------------------------ image output
 
 constexpr int kImgThreads = 256;
+constexpr int kAbsmaxBlocks = 1024;
 
 __global__ void zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
 
-// max |z| over the image (img_output.cpp:24); |z| >= 0, so the f64 bit pattern orders like the value
+// max |z| over the image (img_output.cpp:24); |z| >= 0, so the f64 bit pattern orders like the value.
+// The correctly rounded sqrt is monotone, so each block takes the largest z.x^2 + z.y^2 and one sqrt
+// of it: the same bits as the largest sqrt. Atomics on one address are served one at a time (one
+// per block of 4096 held the kernel at 1.3 TB/s), so the launch has at most kAbsmaxBlocks blocks and
+// a block issues its atomicMax only when its value exceeds the maximum it reads first.
 __global__ __launch_bounds__(kImgThreads) void absmax_kernel(const double2 *__restrict__ img,
                                                              int64_t n,
                                                              unsigned long long *__restrict__ out) {
     __shared__ double red[kImgThreads / 64];
     double m = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {  // four loads in flight per thread
+        double2 z[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) z[k] = img[i + k * stride];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double a2 = z[k].x * z[k].x + z[k].y * z[k].y;
+            m = a2 > m ? a2 : m;
+        }
+    }
+    for (; i < n; i += stride) {
         const double2 z = img[i];
-        const double a = sqrt(z.x * z.x + z.y * z.y);
-        m = a > m ? a : m;
+        const double a2 = z.x * z.x + z.y * z.y;
+        m = a2 > m ? a2 : m;
     }
     for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_down(m, off, 64);
@@ -1021,7 +1037,10 @@ __global__ __launch_bounds__(kImgThreads) void absmax_kernel(const double2 *__re
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < kImgThreads / 64; ++w) m = red[w] > m ? red[w] : m;
-        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+        // most blocks' maxima are below the running one by now: skip their atomics
+        const unsigned long long b = (unsigned long long)__double_as_longlong(sqrt(m));
+        if (b > __hip_atomic_load(out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(out, b);
     }
 }
 
@@ -1302,8 +1321,8 @@ hipError_t launch_img_output(const double2 *img, int64_t npix, double *dB, uint8
     const int64_t want = (npix + kImgThreads - 1) / kImgThreads;
     const int blocks = (int)(want < 4096 ? want : 4096);
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, stream, scratch_max);
-    hipLaunchKernelGGL(absmax_kernel, dim3(blocks), dim3(kImgThreads), 0, stream, img, npix,
-                       scratch_max);
+    hipLaunchKernelGGL(absmax_kernel, dim3(std::min(blocks, kAbsmaxBlocks)), dim3(kImgThreads), 0,
+                       stream, img, npix, scratch_max);
     const double ln10f = (double)std::log(10.0f);
     hipLaunchKernelGGL(db_u8_kernel, dim3(blocks), dim3(kImgThreads), 0, stream, img, npix,
                        scratch_max, ln10f, dB, u8, lo, hi);

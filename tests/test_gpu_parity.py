@@ -361,11 +361,12 @@ def test_pixels_beyond_row_cover_take_exact_path(sar, orc, scenes, torch_dev):
     assert e < TOL
 
 
-def test_near_field_takes_direct_form(sar, orc, scenes, torch_dev):
+def test_near_field_takes_direct_form(sar, orc, scenes, pipe, torch_dev):
     """A platform 36 m from the scene centre. Every pixel lies inside the rows' cover, so the fast
     loop runs, but the range expansion about the tile centres fails its error bound by six orders of
     magnitude (|delta|^4 / (4 d^3) ~ 1e-2 row elements, 2.75 phase revolutions per element): the
-    tiles must take the direct form, or the image would be off by ~0.1 rad of phase."""
+    tiles must take the direct form, or the image would be off by ~0.1 rad of phase. Both the
+    pixel_locs kernel (through backprojection()) and the grid-mode kernel (a device plan)."""
     sc = scenes.circular_arc_scene(npulses=48, nsamples=64, f_c=9.6e9, df=15e6, radius=30.0,
                                    altitude=20.0, arc_deg=40.0, ntargets=5, patch=4.0, snr_db=20.0,
                                    taylor=17, upsample=2, seed=91, name="near")
@@ -388,6 +389,16 @@ def test_near_field_takes_direct_form(sar, orc, scenes, torch_dev):
     ref = orc.backprojection(phs, sc.k_r, 17, n_fft, sc.delta_r, nu, nv, sc.pos, pl)
     e = rel_l2(img, ref)
     print(f"near field: d_min {d_min:.0f} row elements, rel_l2 {e:.3e}")
+    assert e < TOL
+    grid = pipe.scene_grid(sc)
+    assert np.array_equal(grid["pixel_locs"], pl)
+    ds = pipe.DeviceScene(sc, dict(grid, pixel_locs=None), phs_host=phs)
+    assert ds.grid_mode
+    out = ds.output()
+    ds.run(ds.plan(), out)
+    torch_dev.cuda.synchronize()
+    e = rel_l2(out.cpu().numpy(), ref)
+    print(f"near field, grid mode: rel_l2 {e:.3e}")
     assert e < TOL
 
 

@@ -579,6 +579,14 @@ __device__ inline void accumulate_pulses(const BpArgs &a, const PixTerms<PPT> &P
 // pulse each, while the workgroup consumes the other buffer; one barrier per block.
 constexpr int kExpBlock = 128;
 constexpr double kExpandTol = 1e-8;
+constexpr double kRowMagic = 1572864.0;  // 1.5 * 2^20
+static_assert(kMaxFft + kMaxFft / 2 + 2 < (1 << 18), "rows fit the fixed-point index");
+// Row coordinates in fixed point: the expansion's T carries 1.5 * 2^20, so that the f64 result
+// y = t + 1.5 * 2^20 (t in [-2^19, 2^19)) has exponent 20 and its mantissa is (t + 2^19) * 2^32: the
+// high word's low 19 bits hold floor(t) for t >= 0 (a t in [-2^18, 0) gives an index of at least
+// 2^18, beyond every row, which the buffer's bounds check reads as zeros) and the low word holds
+// fract(t) * 2^32. One AND and one conversion instead of three quarter-rate conversions (cvt_i32,
+// fract, cvt_f32); t keeps 2^-32 of a row element (2e-10; the phase 1e-9 rad).
 static_assert(kExpBlock % kBpPulseBlock == 0 && kExpBlock <= kBpThreads,
               "a block of pulses holds whole flush periods and is filled in one pass");
 
@@ -609,7 +617,7 @@ __device__ inline void fill_expansion(const BpArgs &a, TileShared &sh, int blk, 
     const double id = 1.0 / d;
     const double h = 0.5 * id;
     double *k = sh.k[buf][i];
-    k[0] = c.B - d;
+    k[0] = (c.B - d) + kRowMagic;
     if (GRID) {
         const double s = id * inv_lsr;
         k[1] = (Rx * a.g.uh[0] + Ry * a.g.uh[1] + Rz * a.g.uh[2]) * s;
@@ -662,9 +670,10 @@ __device__ inline void gather_expanded(const BpArgs &a, const ExpTerms<PPT> &E, 
     for (int j = 0; j < PPT; ++j) {
         const double q = __builtin_fma(-pr[j], pr[j], E.dd[j]);
         const double m = __builtin_fma(k.g, pr[j], k.h);
-        const double t = __builtin_fma(-q, m, k.T - pr[j]);
-        ii[j] = trunc_i32(t);
-        g.ww[j] = (float)__builtin_amdgcn_fract(t);
+        const double y = __builtin_fma(-q, m, k.T - pr[j]);  // t + kRowMagic
+        const u32x2 b = __builtin_bit_cast(u32x2, y);
+        ii[j] = (int)(b.y & 0x7FFFFu);                        // floor(t)
+        g.ww[j] = (float)b.x * 0x1p-32f;                       // fract(t)
     }
 #pragma unroll
     for (int j = 0; j < PPT; ++j) g.e[j] = loadq(qbase, row, ii[j]);

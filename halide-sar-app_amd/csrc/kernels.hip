@@ -58,12 +58,15 @@ struct QPairF16 {
     __half2 q, dq;
 };
 
+// f32 rows hold B scaled by 2^-32 (exact), so that the accumulation can weight it by fract(t) * 2^32,
+// the integer its fixed-point row coordinate yields, without a multiply; f16 rows hold B itself.
+constexpr float kF32RowBScale = 0x1p-32f;
 __device__ inline void storeq(QPairF32 *p, float2 q, float2 dq) {
     QPairF32 v;
     v.qr = q.x;
     v.qi = q.y;
-    v.dr = dq.x;
-    v.di = dq.y;
+    v.dr = dq.x * kF32RowBScale;
+    v.di = dq.y * kF32RowBScale;
     *p = v;
 }
 __device__ inline void storeq(QPairF16 *p, float2 q, float2 dq) {
@@ -92,6 +95,12 @@ __device__ inline float4 loadq(const QPairF16 *, __amdgpu_buffer_rsrc_t r, int i
     const u32x2 v = struct_load_b64(r, idx, 0, 0, 0);
     const f16x4 h = __builtin_bit_cast(f16x4, v);
     return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
+}
+
+// Interpolation weights are carried in units of the rows' B scale: w * 2^32 for f32 rows, w for f16.
+template <typename QT>
+constexpr float weight_unit() {
+    return sizeof(QT) == sizeof(QPairF16) ? 1.0f : 0x1p32f;
 }
 
 // trunc(t): v_cvt_i32_f64 saturates out-of-range values and gives 0 for NaN.
@@ -490,11 +499,11 @@ __device__ inline void gather_pulse(const BpArgs &a, const PixTerms<PPT> &P, int
         const double t = row_coord(sq[j], c.B);
         if (FAST) {
             ii[j] = trunc_i32(t);
-            g.ww[j] = (float)__builtin_amdgcn_fract(t);
+            g.ww[j] = (float)__builtin_amdgcn_fract(t) * weight_unit<QT>();
         } else {
             ii[j] = trunc_clamp(t, a.len - 2);
             const double w = t - (double)ii[j];
-            g.ww[j] = (float)w;
+            g.ww[j] = (float)w * weight_unit<QT>();
             g.xx[j] = (float)__builtin_amdgcn_fract(a.hdr->kappa * w);
         }
     }
@@ -502,7 +511,8 @@ __device__ inline void gather_pulse(const BpArgs &a, const PixTerms<PPT> &P, int
     for (int j = 0; j < PPT; ++j) g.e[j] = loadq(qbase, row, ii[j]);
 }
 
-// term = (A + w B) exp(2 pi i x) added to the f32 partial sums.
+// term = (A + w B) exp(2 pi i x) added to the f32 partial sums; ww and the stored B carry inverse
+// scales (weight_unit), and on the fast path x = kappa_f ww with kappa_f = kappa / weight_unit.
 template <typename QT, int PPT, bool FAST>
 __device__ inline void add_terms(const Gather<PPT> &g, float kappa_f, float (&sr)[PPT],
                                  float (&si)[PPT]) {
@@ -537,7 +547,7 @@ template <typename QT, int PPT, bool FAST, bool GRID>
 __device__ inline void accumulate_pulses(const BpArgs &a, const PixTerms<PPT> &P,
                                          double (&accr)[PPT], double (&acci)[PPT]) {
     const int np = a.np;
-    const float kappa_f = (float)a.hdr->kappa;
+    const float kappa_f = (float)a.hdr->kappa / weight_unit<QT>();
     float sr[PPT], si[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) sr[j] = si[j] = 0.f;
@@ -673,7 +683,7 @@ __device__ inline void gather_expanded(const BpArgs &a, const ExpTerms<PPT> &E, 
         const double y = __builtin_fma(-q, m, k.T - pr[j]);  // t + kRowMagic
         const u32x2 b = __builtin_bit_cast(u32x2, y);
         ii[j] = (int)(b.y & 0x7FFFFu);                        // floor(t)
-        g.ww[j] = (float)b.x * 0x1p-32f;                       // fract(t)
+        g.ww[j] = (float)b.x * (0x1p-32f * weight_unit<QT>());  // fract(t) in weight units
     }
 #pragma unroll
     for (int j = 0; j < PPT; ++j) g.e[j] = loadq(qbase, row, ii[j]);
@@ -687,7 +697,7 @@ __device__ inline void accumulate_pulses_expanded(const BpArgs &a, const ExpTerm
                                                   TileShared &sh, double (&accr)[PPT],
                                                   double (&acci)[PPT]) {
     const int np = a.np;
-    const float kappa_f = (float)a.hdr->kappa;
+    const float kappa_f = (float)a.hdr->kappa / weight_unit<QT>();
     float sr[PPT], si[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) sr[j] = si[j] = 0.f;
@@ -969,6 +979,16 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, Til
     }
 }
 
+// Waves per SIMD the f32-row kernels are built for (at most 64, 96 and 128 VGPRs): the register
+// budget of a whole kernel is set by its largest loop, and the exact loop for tiles beyond the rows'
+// cover, which almost never runs, must not take occupancy from the others (left to itself it took
+// 160 VGPRs at 4 pixels per thread). Its spills stay in that loop; the expansion and direct fast
+// loops have none. The f16-row kernels keep the compiler's own allocation (1: no minimum).
+template <typename QT>
+constexpr int waves_per_simd(int ppt) {
+    return sizeof(QT) == sizeof(QPairF16) ? 1 : ppt <= 2 ? 8 : ppt == 3 ? 5 : 4;
+}
+
 // Grid mode is a kernel template parameter (not a branch in one kernel): each variant gets its
 // own register allocation, so the grid kernels keep the occupancy their smaller state allows.
 // One workgroup per tile (a 1-D grid of all the tiles), each XCD taking a band of columns or of rows
@@ -978,7 +998,8 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, Til
 // columns, the orientation of the reference's image plane (u_hat along range). Every workgroup of
 // a launch takes the same decision, so the order stays a bijection.
 template <typename QT, int PPT, bool GRID>
-__global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(waves_per_simd<QT>(PPT))))
+void accumulate_kernel(BpArgs a) {
     __shared__ TileShared sh;
     constexpr int th = 16 * PPT;
     const int tiles_x = (a.width + 15) / 16, tiles_y = (a.height + th - 1) / th;
@@ -996,7 +1017,8 @@ __global__ __launch_bounds__(kBpThreads) void accumulate_kernel(BpArgs a) {
 // tiles[xcd_rank(b, ntiles)], so each XCD takes a contiguous run of the list and reads the profile
 // rows of few jobs. All the jobs of a launch share the grid mode.
 template <typename QT, int PPT, bool GRID>
-__global__ __launch_bounds__(kBpThreads) void accumulate_fused_kernel(
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(waves_per_simd<QT>(PPT))))
+void accumulate_fused_kernel(
     const BpArgs *__restrict__ jobs, const FusedTile *__restrict__ tiles, int ntiles) {
     __shared__ TileShared sh;
     const FusedTile t = tiles[xcd_rank((int)blockIdx.x, ntiles)];

@@ -25,8 +25,9 @@
 // list when the scheduler sees its download complete.
 //
 // A per-shape-class autotuner (shape class = N_fft, range-profile format, log2 of pulses and of
-// pixels) times the tile candidates on the first job of a class, over a band of its rows, and uses
-// the fastest for later jobs of that class.
+// pixels) times the tile candidates on the first job of a class, over a band of its rows, each twice
+// in mirrored order, and uses the fastest for later jobs of that class; until the timings are in,
+// jobs use the library's size-based default tile.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -473,7 +474,9 @@ int run_alone(sarbp_server *s, Job &j) {
         const int64_t row_pp = (int64_t)j.d.nu * j.d.npulses;
         const int band = (int)std::min<int64_t>(
             j.d.nv, std::max<int64_t>(192, (4000000000LL / row_pp) / 192 * 192));
-        for (int ppt : {4, 3, 2}) {
+        // each candidate twice, in mirrored order, so that work on other streams that overlaps one
+        // probe does not decide the class; the faster of its two times counts
+        for (int ppt : {4, 3, 2, 2, 3, 4}) {
             hipEvent_t a, b;
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
@@ -486,7 +489,7 @@ int run_alone(sarbp_server *s, Job &j) {
             tu.probes.push_back({ppt, {a, b}});
             if (e != hipSuccess) break;
         }
-        j.ppt = 2;
+        j.ppt = default_tile_ppt(j.args.width, j.args.height);
     }
     if (e != hipSuccess) {
         set_error("job %llu: accumulation failed: %s", (unsigned long long)j.id,
@@ -626,17 +629,21 @@ void scheduler(sarbp_server *s) {
             bool done = true;
             for (auto &pr : tu.probes) done = done && event_done(pr.second.second);
             if (!done) continue;
-            float best_ms = 1e30f;
+            std::map<int, float> fastest;  // tile_ppt -> its faster probe
             for (auto &pr : tu.probes) {
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, pr.second.first, pr.second.second);
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    tu.best = pr.first;
-                }
+                auto it = fastest.find(pr.first);
+                if (it == fastest.end() || ms < it->second) fastest[pr.first] = ms;
                 (void)hipEventDestroy(pr.second.first);
                 (void)hipEventDestroy(pr.second.second);
             }
+            float best_ms = 1e30f;
+            for (auto &kv2 : fastest)
+                if (kv2.second < best_ms) {
+                    best_ms = kv2.second;
+                    tu.best = kv2.first;
+                }
             tu.probes.clear();
             tu.running = false;
         }
@@ -663,7 +670,7 @@ void scheduler(sarbp_server *s) {
             j.dev = s->arena_base + off;
             s->used_bytes += (int64_t)j.ar.total;
             Tuning &tu = s->tuning[j.cls];
-            j.ppt = tu.best ? tu.best : 2;
+            j.ppt = tu.best ? tu.best : default_tile_ppt(j.d.nu, j.d.nv);
             // a small job of an untuned class runs alone once, on a big stream, to be timed
             const bool fuse = j.small && (tu.best != 0 || !s->cfg.autotune);
             j.small = fuse;

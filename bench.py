@@ -65,8 +65,8 @@ def main():
                     help="Poisson arrival rate as a fraction of the measured capacity")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
+        print(f"bench.py: this library runs on one GPU only (asked for --gpus {args.gpus}); "
+              "sharding over several GPUs is not built", file=sys.stderr)
         sys.exit(2)
 
     import torch

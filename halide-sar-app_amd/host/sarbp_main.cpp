@@ -138,7 +138,7 @@ int main(int argc, char **argv) {
         return EXIT_FAILURE;
     } else if (sched == "ritsar" || sched == "ritsar-s" || sched == "ritsar-p" ||
                sched == "ritsar-vp" || sched == "auto-m16") {
-        std::cerr << "Schedule " << sched << " is not part of this build" << std::endl;
+        std::cerr << "Schedule " << sched << " is not part of this library" << std::endl;
         return EXIT_FAILURE;
     } else {
         std::cerr << "Unknown schedule: " << sched << std::endl;

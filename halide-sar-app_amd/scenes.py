@@ -113,7 +113,7 @@ def linear_spotlight_scene(npulses=2000, nsamples=2048, f_c=10e9, bw=1e9, stando
 
 
 def config_scene(name: str) -> Scene:
-    """The named benchmark configurations."""
+    """The named benchmark configurations C1, C2 and C3."""
     if name == "C1":
         return circular_arc_scene()
     if name == "C2":

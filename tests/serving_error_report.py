@@ -1,7 +1,7 @@
 """Serving form: every job's error against the CPU FP64 oracle (test tooling; the oracle is only the
 checker).
 
-    python tests/serving_error_report.py --config C5 > gpurun_out/serving_error_C5.jsonl
+    python tests/serving_error_report.py --config C5 > serving_error_C5.jsonl
 
 Runs a job stream (C4 or C5 as `halide-sar-app_amd/serving.py` builds them) through the serving
 engine: a warm-up pass that also tunes every shape class, then one pass with all jobs submitted at

@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the C ABI) with the CPU FP64 oracle.
 
 Tolerance: relative L2 error <= 1e-4 over complex pixels (||img - oracle|| / ||oracle||), the
-bar the north-star metric states; the measured error is ~1e-6 and is printed. Integer outputs (u8)
+bar the benchmark metric states; the measured error is ~1e-6 and is printed. Integer outputs (u8)
 are compared exactly given identical inputs.
 """
 import importlib

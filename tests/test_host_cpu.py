@@ -215,7 +215,7 @@ def test_cli_options_and_constraints(tmp_path):
     r = _run_sarbp(*base, "-d", "-30", "-D", "0", "-s", "nope")
     assert r.returncode == 1 and "Unknown schedule: nope" in r.stderr
     r = _run_sarbp(*base, "--db-min=-30", "--db-max=0", "--schedule=ritsar")
-    assert r.returncode == 1 and "not part of this build" in r.stderr
+    assert r.returncode == 1 and "not part of this library" in r.stderr
 
 
 def test_cli_reports_loader_errors(tmp_path):

@@ -159,9 +159,10 @@ int32_t sarbp_plan_tile_ppt(const sarbp_plan *p);
 /* Grid mode: the plan's pixel locations come from the image grid (img_plane.cpp:77-113:
  * r(x, y) = (0 + u_hat u[x]) + v_hat v[y], u[x] = (x + floor(-nu/2)) d_u, v likewise, nu and nv from
  * the plan) and are generated inside the accumulation kernel, which then needs no r buffer (r_dev
- * may be NULL in sarbp_plan_execute / sarbp_plan_accumulate). Within a thread's column the kernel
- * derives the squared range of pixels 16, 32, 48 rows further from the first one's, so the image
- * agrees with the r-buffer path to rounding (~1e-8 relative), not bit for bit. The plan's output
+ * may be NULL in sarbp_plan_execute / sarbp_plan_accumulate). The kernel forms ranges from the
+ * grid offsets of each pixel from its tile's centre (or, where that expansion's error bound fails,
+ * derives the squared range of a thread's later pixels from its first one's), so the image agrees
+ * with the r-buffer path to rounding (~1e-8 relative), not bit for bit. The plan's output
  * region must lie inside the image (constraint_violated otherwise). u_hat or v_hat NULL switches
  * back to r. */
 int sarbp_plan_set_grid(sarbp_plan *p, double d_u, double d_v, const double *u_hat,

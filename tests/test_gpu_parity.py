@@ -402,6 +402,44 @@ def test_near_field_takes_direct_form(sar, orc, scenes, pipe, torch_dev):
     assert e < TOL
 
 
+@pytest.mark.parametrize("case", range(12))
+def test_random_geometries(sar, orc, scenes, pipe, torch_dev, case):
+    """Seeded scenes with the platform 25 m to 40 km from the scene centre, 48-128 samples, 8-47
+    pulses, tile heights 32, 48 and 64: on half of them every tile takes the range expansion, on the
+    others some or all tiles fail its error bound and take the direct form. The pixel_locs kernel
+    (backprojection()) over the whole image and the grid-mode kernel (a device plan) over a ragged
+    region, both against the oracle."""
+    c = 299792458.0
+    rng = np.random.default_rng(4400 + case)
+    radius = float(np.exp(rng.uniform(np.log(25.0), np.log(40000.0))))
+    altitude = radius * float(rng.uniform(0.2, 0.9))
+    ns = int(rng.choice([48, 64, 96, 128]))
+    np_ = int(rng.integers(8, 48))
+    df = float(rng.uniform(5e6, 20e6))
+    arc = float(rng.uniform(2.0, 30.0))
+    ppt = int(rng.choice([2, 3, 4]))
+    sc = scenes.circular_arc_scene(npulses=np_, nsamples=ns, f_c=9.6e9, df=df, radius=radius,
+                                   altitude=altitude, arc_deg=arc, ntargets=4,
+                                   patch=0.3 * c / (2 * df), snr_db=30.0, taylor=17, upsample=2,
+                                   seed=4400 + case, name="random")
+    phs = scenes.simulate_phs_numpy(sc)
+    grid = pipe.scene_grid(sc)
+    nu, nv, pl = grid["nu"], grid["nv"], grid["pixel_locs"]
+    n_fft = sar.nfft(ns, 2)
+    ref = orc.backprojection(phs, sc.k_r, 17, n_fft, sc.delta_r, nu, nv, sc.pos, pl)
+    e = rel_l2(run_abi(sar, phs, sc.k_r, 17, n_fft, sc.delta_r, nu, nv, sc.pos, pl), ref)
+    x0, y0 = int(rng.integers(0, nu // 4)), int(rng.integers(0, nv // 4))
+    w, h = int(rng.integers(nu // 2, nu - x0 + 1)), int(rng.integers(nv // 2, nv - y0 + 1))
+    ds = pipe.DeviceScene(sc, dict(grid, pixel_locs=None), phs_host=phs)
+    out = ds.output((x0, y0, w, h))
+    ds.run(ds.plan(region=(x0, y0, w, h), tile_ppt=ppt), out)
+    torch_dev.cuda.synchronize()
+    eg = rel_l2(out.cpu().numpy(), ref[y0:y0 + h, x0:x0 + w])
+    print(f"case {case}: range {radius:.0f} m, ns {ns}, np {np_}, {nu}x{nv}, region "
+          f"{(x0, y0, w, h)} at {16 * ppt} rows: rel_l2 {e:.2e} (pixel_locs), {eg:.2e} (grid)")
+    assert e < TOL and eg < TOL
+
+
 @pytest.mark.parametrize("ppt", [2, 3, 4])
 def test_grid_mode_matches_pixel_locs(sar, orc, scenes, pipe, torch_dev, ppt):
     """Grid mode (pixel locations generated in the accumulation kernel, squared ranges of a

@@ -578,15 +578,16 @@ __device__ inline void accumulate_pulses(const BpArgs &a, const PixTerms<PPT> &P
 // (the Taylor series of d_c sqrt(1 + 2 p / d_c + |delta|^2 / d_c^2) to third order in
 // eps = |delta| / d_c), with |E| <= d_c eps^4 / 8 for eps <= 0.1 (checked against 40-digit
 // arithmetic over random directions). So t = T - p - q (h + g p) with T = B_p - d_c and g = -2 h^2:
-// per pixel-pulse 5 + 1 / PPT f64 operations in grid mode (7 with pixel_locs) and no square root,
-// against 6.5 (8) plus v_rsq_f64 for the direct form. A tile takes this path when
+// per pixel-pulse 7 f64 operations with pixel_locs, and 3 in grid mode, which evaluates the same
+// polynomial in another form (below); no square root, against 8 (6.5 in grid mode) plus v_rsq_f64
+// for the direct form. A tile takes this path when
 // |delta|_max^4 / (4 d_min^3) <= kExpandTol, twice the bound on E, over all of its pixels and pulses
 // (d_min: the least d_c over the pulses); otherwise the direct form. On the C1-C3 scenes the bound
 // is below 2e-10 row elements (3e-9 radians of phase).
 //
-// The per-pulse constants {T, n (grid mode: n.u_hat / lsr, n.v_hat / lsr), h, g} of the tile are
-// built kExpBlock pulses at a time into one of two LDS buffers by the first kExpBlock threads, one
-// pulse each, while the workgroup consumes the other buffer; one barrier per block.
+// With pixel_locs, the per-pulse constants {T, n, h, g} of the tile are built kExpBlock pulses at a
+// time into one of two LDS buffers by the first kExpBlock threads, one pulse each, while the
+// workgroup consumes the other buffer; one barrier per block.
 constexpr int kExpBlock = 128;
 constexpr double kExpandTol = 1e-8;
 constexpr double kRowMagic = 1572864.0;  // 1.5 * 2^20
@@ -600,23 +601,58 @@ static_assert(kMaxFft + kMaxFft / 2 + 2 < (1 << 18), "rows fit the fixed-point i
 static_assert(kExpBlock % kBpPulseBlock == 0 && kExpBlock <= kBpThreads,
               "a block of pulses holds whole flush periods and is filled in one pass");
 
+// Grid mode evaluates the same expansion as a cubic in the pixel's offsets (du, dv) from the centre
+// (metres along u_hat and v_hat): with p = a du + b dv (a = n.u_hat / lsr, b = n.v_hat / lsr) and
+// q = Quu du^2 + 2 Quv du dv + Qvv dv^2 (Quu = sigma |u_hat|^2 - a^2, Quv = sigma u_hat.v_hat - a b,
+// Qvv = sigma |v_hat|^2 - b^2; sigma = inv_lsr^2),
+//   t = T - p - h q - g p q = sum c_ij du^i dv^j (i + j <= 3).
+// A thread's pixels share one column (one du), so per pulse and column the cubic in dv has
+// coefficients C0..C3 (C_k a polynomial of degree 3 - k in du; C3 = c_03 is the same for every
+// column), and each pixel takes three f64 FMAs (Horner in dv) where the form above takes five.
+// Per block of kGridBlock pulses the coefficients are built in two stages that run one block apart:
+// c_ij per pulse (one wave's lanes, one pulse each; the wave rotates with the block), then C0..C2
+// per pulse and column (all threads). One barrier per block. A block of 32 pulses takes 30 KiB of
+// LDS per workgroup (4 or 5 workgroups per CU at 4 or 3 pixels per thread, within 160 KiB); at 2
+// pixels per thread (8 per CU) blocks of 16 take 15 KiB. Halving the block measured 1-2 % slower.
+template <int PPT>
+constexpr int kGridBlock = PPT <= 2 ? 16 : 32;
+static_assert(kBpPulseBlock % kGridBlock<2> == 0 && kBpPulseBlock % kGridBlock<4> == 0 &&
+                  kGridBlock<4> <= 64,
+              "flush periods hold whole blocks; one wave builds a block's c_ij");
+
+template <bool GRID, int PPT>
+struct ExpShared {
+    double k[2][kExpBlock][6];  // per pulse: T, n, h, g
+};
+template <int PPT>
+struct ExpShared<true, PPT> {
+    double c[2][kGridBlock<PPT>][10];       // per pulse: c_00 (+ kRowMagic), c_10, c_20, c_30, c_01,
+                                            // c_11, c_21, c_02, c_12, c_03
+    double2 c01[2][kGridBlock<PPT>][16];    // per pulse and column: (C0, C1)
+    double c2[2][kGridBlock<PPT>][16];      // C2
+    double c3[2][kGridBlock<PPT>];          // C3
+};
+
+template <bool GRID, int PPT>
 struct alignas(16) TileShared {
-    double k[2][kExpBlock][6];  // per pulse: T, n or (a, b, -), h, g
-    double rc[3];               // centre of the expansion (metres)
+    ExpShared<GRID, PPT> x;
+    double rc[3];  // centre of the expansion (metres)
     double dd_max[kBpThreads / 64], d2_min[kBpThreads / 64];
     int far[kBpThreads / 64];
 };
 
-// Per-pixel offsets from the tile's centre. Grid mode: e0[0] = du (metres along u_hat, shared by
-// the thread's pixels, which lie in one column), e1[j] = dv (metres along v_hat). pixel_locs mode:
-// e0, e1, e2 = delta in row units. dd = |delta|^2 in row units.
+// Per-pixel offsets from the tile's centre. Grid mode (used for the tile's error bound only):
+// e0[0] = du (metres along u_hat, shared by the thread's pixels, which lie in one column),
+// e1[j] = dv (metres along v_hat). pixel_locs mode: e0, e1, e2 = delta in row units.
+// dd = |delta|^2 in row units.
 template <int PPT>
 struct ExpTerms {
     double e0[PPT], e1[PPT], e2[PPT], dd[PPT];
 };
 
-template <bool GRID>
-__device__ inline void fill_expansion(const BpArgs &a, TileShared &sh, int blk, int buf) {
+template <int PPT>
+__device__ inline void fill_expansion(const BpArgs &a, TileShared<false, PPT> &sh, int blk,
+                                      int buf) {
     const int i = threadIdx.x, p = blk * kExpBlock + i;
     if (i >= kExpBlock || p >= a.np) return;
     const double inv_lsr = a.hdr->inv_lsr;
@@ -626,20 +662,64 @@ __device__ inline void fill_expansion(const BpArgs &a, TileShared &sh, int blk, 
     const double d = sqrt(Rx * Rx + Ry * Ry + Rz * Rz);
     const double id = 1.0 / d;
     const double h = 0.5 * id;
-    double *k = sh.k[buf][i];
+    double *k = sh.x.k[buf][i];
     k[0] = (c.B - d) + kRowMagic;
-    if (GRID) {
-        const double s = id * inv_lsr;
-        k[1] = (Rx * a.g.uh[0] + Ry * a.g.uh[1] + Rz * a.g.uh[2]) * s;
-        k[2] = (Rx * a.g.vh[0] + Ry * a.g.vh[1] + Rz * a.g.vh[2]) * s;
-        k[3] = 0.0;
-    } else {
-        k[1] = Rx * id;
-        k[2] = Ry * id;
-        k[3] = Rz * id;
-    }
+    k[1] = Rx * id;
+    k[2] = Ry * id;
+    k[3] = Rz * id;
     k[4] = h;
     k[5] = -2.0 * h * h;
+}
+
+// Grid mode, first stage: c_ij of block blk's pulses, one lane per pulse, in wave blk mod 4.
+template <int PPT>
+__device__ inline void fill_grid_cubic(const BpArgs &a, TileShared<true, PPT> &sh, int blk) {
+    constexpr int B = kGridBlock<PPT>;
+    const int i = (int)threadIdx.x - 64 * (blk & 3), p = blk * B + i;
+    if (i < 0 || i >= B || p >= a.np) return;
+    const double inv_lsr = a.hdr->inv_lsr, sigma = inv_lsr * inv_lsr;
+    const PulseConst c = a.pc[p];
+    const double Rx = (sh.rc[0] - c.px) * inv_lsr, Ry = (sh.rc[1] - c.py) * inv_lsr,
+                 Rz = (sh.rc[2] - c.pz) * inv_lsr;
+    const double d = sqrt(Rx * Rx + Ry * Ry + Rz * Rz);
+    const double id = 1.0 / d;
+    const double h = 0.5 * id, g = -2.0 * h * h;
+    const double *uh = a.g.uh, *vh = a.g.vh;
+    const double s = id * inv_lsr;
+    const double na = (Rx * uh[0] + Ry * uh[1] + Rz * uh[2]) * s;
+    const double nb = (Rx * vh[0] + Ry * vh[1] + Rz * vh[2]) * s;
+    const double quu = sigma * (uh[0] * uh[0] + uh[1] * uh[1] + uh[2] * uh[2]) - na * na;
+    const double quv = sigma * (uh[0] * vh[0] + uh[1] * vh[1] + uh[2] * vh[2]) - na * nb;
+    const double qvv = sigma * (vh[0] * vh[0] + vh[1] * vh[1] + vh[2] * vh[2]) - nb * nb;
+    double *k = sh.x.c[blk & 1][i];
+    k[0] = (c.B - d) + kRowMagic;
+    k[1] = -na;
+    k[2] = -h * quu;
+    k[3] = -g * na * quu;
+    k[4] = -nb;
+    k[5] = -2.0 * h * quv;
+    k[6] = -g * (nb * quu + 2.0 * na * quv);
+    k[7] = -h * qvv;
+    k[8] = -g * (2.0 * nb * quv + na * qvv);
+    k[9] = -g * nb * qvv;
+}
+
+// Grid mode, second stage: C0..C3 of block blk's pulses for the tile's 16 columns
+// (du = (x - 7.5) d_u), from the c_ij of the first stage.
+template <int PPT>
+__device__ inline void fill_grid_columns(const BpArgs &a, TileShared<true, PPT> &sh, int blk) {
+    constexpr int B = kGridBlock<PPT>;
+    const int n = min(B, a.np - blk * B), buf = blk & 1;
+    for (int job = threadIdx.x; job < 16 * n; job += kBpThreads) {
+        const int i = job >> 4, x = job & 15;
+        const double *k = sh.x.c[buf][i];
+        const double du = ((double)x - 7.5) * a.g.d_u;
+        const double c0 = __builtin_fma(__builtin_fma(__builtin_fma(k[3], du, k[2]), du, k[1]), du, k[0]);
+        const double c1 = __builtin_fma(__builtin_fma(k[6], du, k[5]), du, k[4]);
+        sh.x.c01[buf][i][x] = make_double2(c0, c1);
+        sh.x.c2[buf][i][x] = __builtin_fma(k[8], du, k[7]);
+        if (x == 0) sh.x.c3[buf][i] = k[9];
+    }
 }
 
 // One pulse's expansion constants from the LDS block (the address is the same in every lane, so
@@ -658,30 +738,17 @@ __device__ inline ExpConst read_expansion(const double *kp) {
     return k;
 }
 
-template <typename QT, int PPT, bool GRID>
-__device__ inline void gather_expanded(const BpArgs &a, const ExpTerms<PPT> &E, const ExpConst &k,
-                                       int p, Gather<PPT> &g) {
+// Index and weight of the thread's pixels from y = t + kRowMagic, then their loads.
+template <typename QT, int PPT>
+__device__ inline void gather_fixed(const BpArgs &a, const double (&y)[PPT], int p, Gather<PPT> &g) {
     constexpr bool kScaled = sizeof(QT) == sizeof(QPairF16);
     const QT *qbase = reinterpret_cast<const QT *>(a.q);
     const __amdgpu_buffer_rsrc_t row = row_rsrc(qbase + (size_t)p * a.len, a.len, (int)sizeof(QT));
     g.qs = kScaled ? a.qscale[p] : 1.0f;
-    double pr[PPT];
-    if (GRID) {
-        const double au = k.n0 * E.e0[0];
-#pragma unroll
-        for (int j = 0; j < PPT; ++j) pr[j] = __builtin_fma(k.n1, E.e1[j], au);
-    } else {
-#pragma unroll
-        for (int j = 0; j < PPT; ++j)
-            pr[j] = __builtin_fma(k.n0, E.e0[j], __builtin_fma(k.n1, E.e1[j], k.n2 * E.e2[j]));
-    }
     int ii[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        const double q = __builtin_fma(-pr[j], pr[j], E.dd[j]);
-        const double m = __builtin_fma(k.g, pr[j], k.h);
-        const double y = __builtin_fma(-q, m, k.T - pr[j]);  // t + kRowMagic
-        const u32x2 b = __builtin_bit_cast(u32x2, y);
+        const u32x2 b = __builtin_bit_cast(u32x2, y[j]);
         ii[j] = (int)(b.y & 0x7FFFFu);                        // floor(t)
         g.ww[j] = (float)b.x * (0x1p-32f * weight_unit<QT>());  // fract(t) in weight units
     }
@@ -689,12 +756,90 @@ __device__ inline void gather_expanded(const BpArgs &a, const ExpTerms<PPT> &E, 
     for (int j = 0; j < PPT; ++j) g.e[j] = loadq(qbase, row, ii[j]);
 }
 
-// The fast pulse loop over the expansion: per block of kExpBlock pulses a barrier, the next block's
-// constants, then the block's pulses pipelined as in accumulate_pulses (flush after every
-// kBpPulseBlock pulses of the whole sequence).
-template <typename QT, int PPT, bool GRID>
+template <typename QT, int PPT>
+__device__ inline void gather_expanded(const BpArgs &a, const ExpTerms<PPT> &E, const ExpConst &k,
+                                       int p, Gather<PPT> &g) {
+    double y[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const double pr = __builtin_fma(k.n0, E.e0[j], __builtin_fma(k.n1, E.e1[j], k.n2 * E.e2[j]));
+        const double q = __builtin_fma(-pr, pr, E.dd[j]);
+        const double m = __builtin_fma(k.g, pr, k.h);
+        y[j] = __builtin_fma(-q, m, k.T - pr);  // t + kRowMagic
+    }
+    gather_fixed<QT, PPT>(a, y, p, g);
+}
+
+// Grid mode: y = C0 + dv (C1 + dv (C2 + dv C3)) for the thread's pixels, with the pulse's C0..C3 of
+// the thread's column.
+template <typename QT, int PPT>
+__device__ inline void gather_grid(const BpArgs &a, const double (&dv)[PPT], const double2 &c01,
+                                   double c2, double c3, int p, Gather<PPT> &g) {
+    double y[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j)
+        y[j] = __builtin_fma(__builtin_fma(__builtin_fma(c3, dv[j], c2), dv[j], c01.y), dv[j], c01.x);
+    gather_fixed<QT, PPT>(a, y, p, g);
+}
+
+// The fast pulse loop over the cubic (grid mode): blocks of kGridBlock pulses; in the period of
+// block b the threads build block b + 2's c_ij and block b + 1's C0..C3, then consume block b's
+// (pipelined as in accumulate_pulses; flush after every kBpPulseBlock pulses of the whole
+// sequence). Each write goes to the buffer of the block two back, whose last reader ran in an
+// earlier period: c_ij of block b + 2 replace those of b (read by the second stage in period
+// b - 1), C of b + 1 those of b - 1 (consumed in period b - 1).
+template <typename QT, int PPT>
+__device__ inline void accumulate_pulses_grid(const BpArgs &a, const double (&dv)[PPT], int col,
+                                              TileShared<true, PPT> &sh, double (&accr)[PPT],
+                                              double (&acci)[PPT]) {
+    constexpr int B = kGridBlock<PPT>;
+    const int np = a.np;
+    const float kappa_f = (float)a.hdr->kappa / weight_unit<QT>();
+    float sr[PPT], si[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) sr[j] = si[j] = 0.f;
+    auto flush = [&] {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            accr[j] += (double)sr[j];
+            acci[j] += (double)si[j];
+            sr[j] = si[j] = 0.f;
+        }
+    };
+    const int nblk = (np + B - 1) / B;
+    fill_grid_cubic<PPT>(a, sh, 0);
+    if (nblk > 1) fill_grid_cubic<PPT>(a, sh, 1);
+    __syncthreads();
+    fill_grid_columns<PPT>(a, sh, 0);
+    for (int b = 0; b < nblk; ++b) {
+        __syncthreads();
+        if (b + 2 < nblk) fill_grid_cubic<PPT>(a, sh, b + 2);
+        if (b + 1 < nblk) fill_grid_columns<PPT>(a, sh, b + 1);
+        const int buf = b & 1, p0 = b * B, p1 = min(np, p0 + B);
+        const double2 *c01 = &sh.x.c01[buf][0][col];
+        const double *c2 = &sh.x.c2[buf][0][col], *c3 = sh.x.c3[buf];
+        Gather<PPT> g0, g1;
+        gather_grid<QT, PPT>(a, dv, c01[0], c2[0], c3[0], p0, g0);
+        for (int p = p0; p + 1 < p1; p += 2) {
+            const int i1 = p + 1 - p0;
+            gather_grid<QT, PPT>(a, dv, c01[16 * i1], c2[16 * i1], c3[i1], p + 1, g1);
+            add_terms<QT, PPT, true>(g0, kappa_f, sr, si);
+            const int pn = min(p + 2, p1 - 1), i2 = pn - p0;
+            gather_grid<QT, PPT>(a, dv, c01[16 * i2], c2[16 * i2], c3[i2], pn, g0);
+            add_terms<QT, PPT, true>(g1, kappa_f, sr, si);
+            if (((p + 2) & (kBpPulseBlock - 1)) == 0) flush();
+        }
+        if ((p1 - p0) & 1) add_terms<QT, PPT, true>(g0, kappa_f, sr, si);  // pulse p1 - 1
+    }
+    flush();
+}
+
+// The fast pulse loop over the expansion (pixel_locs mode): per block of kExpBlock pulses a barrier,
+// the next block's constants, then the block's pulses pipelined as in accumulate_pulses (flush after
+// every kBpPulseBlock pulses of the whole sequence).
+template <typename QT, int PPT>
 __device__ inline void accumulate_pulses_expanded(const BpArgs &a, const ExpTerms<PPT> &E,
-                                                  TileShared &sh, double (&accr)[PPT],
+                                                  TileShared<false, PPT> &sh, double (&accr)[PPT],
                                                   double (&acci)[PPT]) {
     const int np = a.np;
     const float kappa_f = (float)a.hdr->kappa / weight_unit<QT>();
@@ -710,19 +855,19 @@ __device__ inline void accumulate_pulses_expanded(const BpArgs &a, const ExpTerm
         }
     };
     const int nblk = (np + kExpBlock - 1) / kExpBlock;
-    fill_expansion<GRID>(a, sh, 0, 0);
+    fill_expansion<PPT>(a, sh, 0, 0);
     for (int b = 0; b < nblk; ++b) {
         __syncthreads();  // block b's constants written; block b - 1's buffer no longer read
-        if (b + 1 < nblk) fill_expansion<GRID>(a, sh, b + 1, (b + 1) & 1);
-        const double *kb = &sh.k[b & 1][0][0];
+        if (b + 1 < nblk) fill_expansion<PPT>(a, sh, b + 1, (b + 1) & 1);
+        const double *kb = &sh.x.k[b & 1][0][0];
         const int p0 = b * kExpBlock, p1 = min(np, p0 + kExpBlock);
         Gather<PPT> g0, g1;
-        gather_expanded<QT, PPT, GRID>(a, E, read_expansion(kb), p0, g0);
+        gather_expanded<QT, PPT>(a, E, read_expansion(kb), p0, g0);
         for (int p = p0; p + 1 < p1; p += 2) {
-            gather_expanded<QT, PPT, GRID>(a, E, read_expansion(kb + 6 * (p + 1 - p0)), p + 1, g1);
+            gather_expanded<QT, PPT>(a, E, read_expansion(kb + 6 * (p + 1 - p0)), p + 1, g1);
             add_terms<QT, PPT, true>(g0, kappa_f, sr, si);
             const int pn = min(p + 2, p1 - 1);
-            gather_expanded<QT, PPT, GRID>(a, E, read_expansion(kb + 6 * (pn - p0)), pn, g0);
+            gather_expanded<QT, PPT>(a, E, read_expansion(kb + 6 * (pn - p0)), pn, g0);
             add_terms<QT, PPT, true>(g1, kappa_f, sr, si);
             if (((p + 2) & (kBpPulseBlock - 1)) == 0) flush();
         }
@@ -840,8 +985,25 @@ __device__ inline void expansion_terms(const BpArgs &a, int bx, int by, int tx, 
     }
 }
 
+// Grid mode: the thread's pixel offsets dv from the centre (metres along v_hat) and its column of
+// the cubic's coefficients. As in expansion_terms, a pixel outside the region takes pixel 0's row,
+// and a thread whose pixel 0 is outside it takes the tile's first pixel (column 0, row 0).
+template <int PPT>
+__device__ inline void grid_offsets(const BpArgs &a, int bx, int by, int tx, int ty, double (&dv)[PPT],
+                                    int &col) {
+    const bool ok0 = bx + tx < a.width && by + ty < a.height;
+    col = ok0 ? tx : 0;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const bool ok = by + ty + 16 * j < a.height;
+        const double oy = ok0 ? (double)(ty + 16 * (ok ? j : 0)) : 0.0;
+        dv[j] = ((8.0 * PPT - 0.5) - oy) * a.g.d_v;
+    }
+}
+
 template <typename QT, int PPT, bool GRID>
-__device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, TileShared &sh) {
+__device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
+                                            TileShared<GRID, PPT> &sh) {
     const double inv_lsr = a.hdr->inv_lsr;
     // Wave w holds rows 4w..4w+3 (+16 j) of the tile's 16 columns. The memory pipeline serves a
     // gather four lanes at a time and its cost grows with the cache lines each quad of lanes
@@ -946,10 +1108,17 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, Til
 #pragma unroll
     for (int j = 0; j < PPT; ++j) accr[j] = acci[j] = 0.0;
     if (path == 1) {
-        ExpTerms<PPT> E;
-        const double rcs[3] = {sh.rc[0], sh.rc[1], sh.rc[2]};
-        expansion_terms<GRID>(a, bx, by, opaque(tx), opaque(ty), rcs, E);
-        accumulate_pulses_expanded<QT, PPT, GRID>(a, E, sh, accr, acci);
+        if constexpr (GRID) {
+            double dv[PPT];
+            int col;
+            grid_offsets<PPT>(a, bx, by, opaque(tx), opaque(ty), dv, col);
+            accumulate_pulses_grid<QT, PPT>(a, dv, col, sh, accr, acci);
+        } else {
+            ExpTerms<PPT> E;
+            const double rcs[3] = {sh.rc[0], sh.rc[1], sh.rc[2]};
+            expansion_terms<GRID>(a, bx, by, opaque(tx), opaque(ty), rcs, E);
+            accumulate_pulses_expanded<QT, PPT>(a, E, sh, accr, acci);
+        }
     } else {
         PixTerms<PPT> P;
         direct_terms<GRID>(a, bx, by, opaque(tx), opaque(ty), P);
@@ -983,10 +1152,13 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by, Til
 // budget of a whole kernel is set by its largest loop, and the exact loop for tiles beyond the rows'
 // cover, which almost never runs, must not take occupancy from the others (left to itself it took
 // 160 VGPRs at 4 pixels per thread). Its spills stay in that loop; the expansion and direct fast
-// loops have none. The f16-row kernels keep the compiler's own allocation (1: no minimum).
-template <typename QT>
+// loops have none. The f16-row kernels keep the compiler's own allocation (1: no minimum), except
+// the grid kernel at 4 pixels per thread, which fits 128 VGPRs without spills when asked to (left
+// to itself it took 129, and one wave less per SIMD).
+template <typename QT, bool GRID>
 constexpr int waves_per_simd(int ppt) {
-    return sizeof(QT) == sizeof(QPairF16) ? 1 : ppt <= 2 ? 8 : ppt == 3 ? 5 : 4;
+    return sizeof(QT) == sizeof(QPairF16) ? (GRID && ppt == 4 ? 4 : 1)
+                                          : ppt <= 2 ? 8 : ppt == 3 ? 5 : 4;
 }
 
 // Grid mode is a kernel template parameter (not a branch in one kernel): each variant gets its
@@ -998,9 +1170,9 @@ constexpr int waves_per_simd(int ppt) {
 // columns, the orientation of the reference's image plane (u_hat along range). Every workgroup of
 // a launch takes the same decision, so the order stays a bijection.
 template <typename QT, int PPT, bool GRID>
-__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(waves_per_simd<QT>(PPT))))
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(waves_per_simd<QT, GRID>(PPT))))
 void accumulate_kernel(BpArgs a) {
-    __shared__ TileShared sh;
+    __shared__ TileShared<GRID, PPT> sh;
     constexpr int th = 16 * PPT;
     const int tiles_x = (a.width + 15) / 16, tiles_y = (a.height + th - 1) / th;
     bool cols = true;
@@ -1017,10 +1189,10 @@ void accumulate_kernel(BpArgs a) {
 // tiles[xcd_rank(b, ntiles)], so each XCD takes a contiguous run of the list and reads the profile
 // rows of few jobs. All the jobs of a launch share the grid mode.
 template <typename QT, int PPT, bool GRID>
-__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(waves_per_simd<QT>(PPT))))
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(waves_per_simd<QT, GRID>(PPT))))
 void accumulate_fused_kernel(
     const BpArgs *__restrict__ jobs, const FusedTile *__restrict__ tiles, int ntiles) {
-    __shared__ TileShared sh;
+    __shared__ TileShared<GRID, PPT> sh;
     const FusedTile t = tiles[xcd_rank((int)blockIdx.x, ntiles)];
     const BpArgs a = jobs[t.job];
     accumulate_tile_impl<QT, PPT, GRID>(a, t.bx, t.by, sh);

@@ -94,15 +94,22 @@ struct BpArgs {
     int ppt;  // pixels per thread (tile height 16 * ppt)
     int64_t r_min, r_extent;
     int nu, nv, x0, y0, width, height;
+    int full_w, full_h;  // the whole region of the plan or job (a band launch covers part of it)
+    int cubic;           // grid mode: expansion tiles take the cubic form (set by the launchers)
     double2 *out;
     PixelGrid g;  // g.on: locations from the grid, r unused
 };
 hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream);
+// Grid mode: whether a plan's or job's expansion tiles take the cubic form, from its whole region
+// (full_w x full_h at a.ppt), so that every launch of one job, banded, fused or alone, takes the
+// same form. 0 without a grid.
+int grid_cubic(const BpArgs &a);
 // A tile of a fused launch: job index and the tile's top-left output pixel within that job's region.
 struct FusedTile {
     int job, bx, by, pad;
 };
-// All jobs of one fused launch share q_format, ppt and the grid mode (BpArgs::g.on == grid).
+// All jobs of one fused launch share q_format, ppt and the grid mode (BpArgs::g.on == grid); each
+// job's BpArgs::cubic is read as the caller set it (grid_cubic).
 hipError_t launch_accumulate_fused(const BpArgs *jobs_dev, const FusedTile *tiles_dev, int ntiles,
                                    int q_format, int ppt, bool grid, hipStream_t stream);
 // Output rows covered by one accumulation tile for a pixels-per-thread setting (16 columns wide).

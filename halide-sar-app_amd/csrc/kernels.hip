@@ -625,12 +625,19 @@ struct ExpShared {
     double k[2][kExpBlock][6];  // per pulse: T, n, h, g
 };
 template <int PPT>
-struct ExpShared<true, PPT> {
+struct GridCubic {
     double c[2][kGridBlock<PPT>][10];       // per pulse: c_00 (+ kRowMagic), c_10, c_20, c_30, c_01,
                                             // c_11, c_21, c_02, c_12, c_03
     double2 c01[2][kGridBlock<PPT>][16];    // per pulse and column: (C0, C1)
     double c2[2][kGridBlock<PPT>][16];      // C2
     double c3[2][kGridBlock<PPT>];          // C3
+};
+template <int PPT>
+struct ExpShared<true, PPT> {
+    union {
+        double k[2][kExpBlock][6];  // per pulse: T, n.u_hat / lsr, n.v_hat / lsr, -, h, g
+        GridCubic<PPT> cub;
+    };
 };
 
 template <bool GRID, int PPT>
@@ -650,8 +657,8 @@ struct ExpTerms {
     double e0[PPT], e1[PPT], e2[PPT], dd[PPT];
 };
 
-template <int PPT>
-__device__ inline void fill_expansion(const BpArgs &a, TileShared<false, PPT> &sh, int blk,
+template <bool GRID, int PPT>
+__device__ inline void fill_expansion(const BpArgs &a, TileShared<GRID, PPT> &sh, int blk,
                                       int buf) {
     const int i = threadIdx.x, p = blk * kExpBlock + i;
     if (i >= kExpBlock || p >= a.np) return;
@@ -664,9 +671,16 @@ __device__ inline void fill_expansion(const BpArgs &a, TileShared<false, PPT> &s
     const double h = 0.5 * id;
     double *k = sh.x.k[buf][i];
     k[0] = (c.B - d) + kRowMagic;
-    k[1] = Rx * id;
-    k[2] = Ry * id;
-    k[3] = Rz * id;
+    if (GRID) {
+        const double s = id * inv_lsr;
+        k[1] = (Rx * a.g.uh[0] + Ry * a.g.uh[1] + Rz * a.g.uh[2]) * s;
+        k[2] = (Rx * a.g.vh[0] + Ry * a.g.vh[1] + Rz * a.g.vh[2]) * s;
+        k[3] = 0.0;
+    } else {
+        k[1] = Rx * id;
+        k[2] = Ry * id;
+        k[3] = Rz * id;
+    }
     k[4] = h;
     k[5] = -2.0 * h * h;
 }
@@ -691,7 +705,7 @@ __device__ inline void fill_grid_cubic(const BpArgs &a, TileShared<true, PPT> &s
     const double quu = sigma * (uh[0] * uh[0] + uh[1] * uh[1] + uh[2] * uh[2]) - na * na;
     const double quv = sigma * (uh[0] * vh[0] + uh[1] * vh[1] + uh[2] * vh[2]) - na * nb;
     const double qvv = sigma * (vh[0] * vh[0] + vh[1] * vh[1] + vh[2] * vh[2]) - nb * nb;
-    double *k = sh.x.c[blk & 1][i];
+    double *k = sh.x.cub.c[blk & 1][i];
     k[0] = (c.B - d) + kRowMagic;
     k[1] = -na;
     k[2] = -h * quu;
@@ -712,13 +726,13 @@ __device__ inline void fill_grid_columns(const BpArgs &a, TileShared<true, PPT> 
     const int n = min(B, a.np - blk * B), buf = blk & 1;
     for (int job = threadIdx.x; job < 16 * n; job += kBpThreads) {
         const int i = job >> 4, x = job & 15;
-        const double *k = sh.x.c[buf][i];
+        const double *k = sh.x.cub.c[buf][i];
         const double du = ((double)x - 7.5) * a.g.d_u;
         const double c0 = __builtin_fma(__builtin_fma(__builtin_fma(k[3], du, k[2]), du, k[1]), du, k[0]);
         const double c1 = __builtin_fma(__builtin_fma(k[6], du, k[5]), du, k[4]);
-        sh.x.c01[buf][i][x] = make_double2(c0, c1);
-        sh.x.c2[buf][i][x] = __builtin_fma(k[8], du, k[7]);
-        if (x == 0) sh.x.c3[buf][i] = k[9];
+        sh.x.cub.c01[buf][i][x] = make_double2(c0, c1);
+        sh.x.cub.c2[buf][i][x] = __builtin_fma(k[8], du, k[7]);
+        if (x == 0) sh.x.cub.c3[buf][i] = k[9];
     }
 }
 
@@ -756,13 +770,16 @@ __device__ inline void gather_fixed(const BpArgs &a, const double (&y)[PPT], int
     for (int j = 0; j < PPT; ++j) g.e[j] = loadq(qbase, row, ii[j]);
 }
 
-template <typename QT, int PPT>
+template <typename QT, int PPT, bool GRID>
 __device__ inline void gather_expanded(const BpArgs &a, const ExpTerms<PPT> &E, const ExpConst &k,
                                        int p, Gather<PPT> &g) {
     double y[PPT];
+    const double au = GRID ? k.n0 * E.e0[0] : 0.0;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        const double pr = __builtin_fma(k.n0, E.e0[j], __builtin_fma(k.n1, E.e1[j], k.n2 * E.e2[j]));
+        const double pr =
+            GRID ? __builtin_fma(k.n1, E.e1[j], au)
+                 : __builtin_fma(k.n0, E.e0[j], __builtin_fma(k.n1, E.e1[j], k.n2 * E.e2[j]));
         const double q = __builtin_fma(-pr, pr, E.dd[j]);
         const double m = __builtin_fma(k.g, pr, k.h);
         y[j] = __builtin_fma(-q, m, k.T - pr);  // t + kRowMagic
@@ -816,8 +833,8 @@ __device__ inline void accumulate_pulses_grid(const BpArgs &a, const double (&dv
         if (b + 2 < nblk) fill_grid_cubic<PPT>(a, sh, b + 2);
         if (b + 1 < nblk) fill_grid_columns<PPT>(a, sh, b + 1);
         const int buf = b & 1, p0 = b * B, p1 = min(np, p0 + B);
-        const double2 *c01 = &sh.x.c01[buf][0][col];
-        const double *c2 = &sh.x.c2[buf][0][col], *c3 = sh.x.c3[buf];
+        const double2 *c01 = &sh.x.cub.c01[buf][0][col];
+        const double *c2 = &sh.x.cub.c2[buf][0][col], *c3 = sh.x.cub.c3[buf];
         Gather<PPT> g0, g1;
         gather_grid<QT, PPT>(a, dv, c01[0], c2[0], c3[0], p0, g0);
         for (int p = p0; p + 1 < p1; p += 2) {
@@ -834,12 +851,12 @@ __device__ inline void accumulate_pulses_grid(const BpArgs &a, const double (&dv
     flush();
 }
 
-// The fast pulse loop over the expansion (pixel_locs mode): per block of kExpBlock pulses a barrier,
+// The fast pulse loop over the expansion in the form above: per block of kExpBlock pulses a barrier,
 // the next block's constants, then the block's pulses pipelined as in accumulate_pulses (flush after
 // every kBpPulseBlock pulses of the whole sequence).
-template <typename QT, int PPT>
+template <typename QT, int PPT, bool GRID>
 __device__ inline void accumulate_pulses_expanded(const BpArgs &a, const ExpTerms<PPT> &E,
-                                                  TileShared<false, PPT> &sh, double (&accr)[PPT],
+                                                  TileShared<GRID, PPT> &sh, double (&accr)[PPT],
                                                   double (&acci)[PPT]) {
     const int np = a.np;
     const float kappa_f = (float)a.hdr->kappa / weight_unit<QT>();
@@ -855,19 +872,19 @@ __device__ inline void accumulate_pulses_expanded(const BpArgs &a, const ExpTerm
         }
     };
     const int nblk = (np + kExpBlock - 1) / kExpBlock;
-    fill_expansion<PPT>(a, sh, 0, 0);
+    fill_expansion<GRID, PPT>(a, sh, 0, 0);
     for (int b = 0; b < nblk; ++b) {
         __syncthreads();  // block b's constants written; block b - 1's buffer no longer read
-        if (b + 1 < nblk) fill_expansion<PPT>(a, sh, b + 1, (b + 1) & 1);
+        if (b + 1 < nblk) fill_expansion<GRID, PPT>(a, sh, b + 1, (b + 1) & 1);
         const double *kb = &sh.x.k[b & 1][0][0];
         const int p0 = b * kExpBlock, p1 = min(np, p0 + kExpBlock);
         Gather<PPT> g0, g1;
-        gather_expanded<QT, PPT>(a, E, read_expansion(kb), p0, g0);
+        gather_expanded<QT, PPT, GRID>(a, E, read_expansion(kb), p0, g0);
         for (int p = p0; p + 1 < p1; p += 2) {
-            gather_expanded<QT, PPT>(a, E, read_expansion(kb + 6 * (p + 1 - p0)), p + 1, g1);
+            gather_expanded<QT, PPT, GRID>(a, E, read_expansion(kb + 6 * (p + 1 - p0)), p + 1, g1);
             add_terms<QT, PPT, true>(g0, kappa_f, sr, si);
             const int pn = min(p + 2, p1 - 1);
-            gather_expanded<QT, PPT>(a, E, read_expansion(kb + 6 * (pn - p0)), pn, g0);
+            gather_expanded<QT, PPT, GRID>(a, E, read_expansion(kb + 6 * (pn - p0)), pn, g0);
             add_terms<QT, PPT, true>(g1, kappa_f, sr, si);
             if (((p + 2) & (kBpPulseBlock - 1)) == 0) flush();
         }
@@ -1003,7 +1020,7 @@ __device__ inline void grid_offsets(const BpArgs &a, int bx, int by, int tx, int
 
 template <typename QT, int PPT, bool GRID>
 __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
-                                            TileShared<GRID, PPT> &sh) {
+                                            TileShared<GRID, PPT> &sh, bool cubic) {
     const double inv_lsr = a.hdr->inv_lsr;
     // Wave w holds rows 4w..4w+3 (+16 j) of the tile's 16 columns. The memory pipeline serves a
     // gather four lanes at a time and its cost grows with the cache lines each quad of lanes
@@ -1107,18 +1124,18 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
     double accr[PPT], acci[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) accr[j] = acci[j] = 0.0;
-    if (path == 1) {
+    if (path == 1 && GRID && cubic) {
         if constexpr (GRID) {
             double dv[PPT];
             int col;
             grid_offsets<PPT>(a, bx, by, opaque(tx), opaque(ty), dv, col);
             accumulate_pulses_grid<QT, PPT>(a, dv, col, sh, accr, acci);
-        } else {
-            ExpTerms<PPT> E;
-            const double rcs[3] = {sh.rc[0], sh.rc[1], sh.rc[2]};
-            expansion_terms<GRID>(a, bx, by, opaque(tx), opaque(ty), rcs, E);
-            accumulate_pulses_expanded<QT, PPT>(a, E, sh, accr, acci);
         }
+    } else if (path == 1) {
+        ExpTerms<PPT> E;
+        const double rcs[3] = {sh.rc[0], sh.rc[1], sh.rc[2]};
+        expansion_terms<GRID>(a, bx, by, opaque(tx), opaque(ty), rcs, E);
+        accumulate_pulses_expanded<QT, PPT, GRID>(a, E, sh, accr, acci);
     } else {
         PixTerms<PPT> P;
         direct_terms<GRID>(a, bx, by, opaque(tx), opaque(ty), P);
@@ -1168,7 +1185,8 @@ constexpr int waves_per_simd(int ppt) {
 // height. Grid mode measures that with the mid pulse's U and V (the gradient of the squared range
 // along u and v at the scene origin, where image planes are centred); pixel_locs launches use
 // columns, the orientation of the reference's image plane (u_hat along range). Every workgroup of
-// a launch takes the same decision, so the order stays a bijection.
+// a launch takes the same decision, so the order stays a bijection. a.cubic (grid mode): expansion
+// tiles take the cubic form (accumulate_pulses_grid) rather than the p, q form.
 template <typename QT, int PPT, bool GRID>
 __global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(waves_per_simd<QT, GRID>(PPT))))
 void accumulate_kernel(BpArgs a) {
@@ -1182,7 +1200,7 @@ void accumulate_kernel(BpArgs a) {
     }
     int tx, ty;
     xcd_band_tile((int)blockIdx.x, tiles_x, tiles_y, cols, tx, ty);
-    accumulate_tile_impl<QT, PPT, GRID>(a, tx * 16, ty * th, sh);
+    accumulate_tile_impl<QT, PPT, GRID>(a, tx * 16, ty * th, sh, a.cubic != 0);
 }
 
 // Several jobs in one launch over a tile list (job by job, row-major): block b computes tile
@@ -1195,7 +1213,7 @@ void accumulate_fused_kernel(
     __shared__ TileShared<GRID, PPT> sh;
     const FusedTile t = tiles[xcd_rank((int)blockIdx.x, ntiles)];
     const BpArgs a = jobs[t.job];
-    accumulate_tile_impl<QT, PPT, GRID>(a, t.bx, t.by, sh);
+    accumulate_tile_impl<QT, PPT, GRID>(a, t.bx, t.by, sh, a.cubic != 0);
 }
 
 // ------------------------------------------------------------------ image output
@@ -1419,15 +1437,42 @@ hipError_t launch_range_profiles(const void *phs, const float *k_r, const double
     return hipGetLastError();
 }
 
+// The cubic form builds its coefficients every kGridBlock pulses behind a barrier, a latency that
+// other workgroups on the CU hide once the region holds at least two device-fulls of tiles; smaller
+// regions (C1: 512 tiles of 16 x 32, 2 per CU) ran 19 % slower with it than with the p, q form,
+// whose blocks are 128 pulses.
+int grid_cubic(const BpArgs &a) {
+    if (!a.g.on) return 0;
+    const int th = tile_height(a.ppt), ppt = th / 16;
+    const int64_t tiles = (int64_t)((a.full_w + 15) / 16) * ((a.full_h + th - 1) / th);
+    const int q_format = a.q_format;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return 0;
+    static std::mutex m;
+    static std::map<std::pair<int, int>, int> resident;  // (device, 8 q_format + ppt) -> tiles
+    int full;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        const auto key = std::make_pair(device, 8 * q_format + ppt);
+        auto it = resident.find(key);
+        if (it == resident.end())
+            it = resident.emplace(key, accumulate_resident_tiles(device, q_format, ppt, true)).first;
+        full = it->second;
+    }
+    return full > 0 && tiles >= 2 * (int64_t)full;
+}
+
 template <int PPT, bool GRID>
 static hipError_t launch_accumulate_ppt(const BpArgs &a, hipStream_t stream) {
     const dim3 grid((unsigned)(((a.width + 15) / 16) * ((a.height + 16 * PPT - 1) / (16 * PPT))));
+    BpArgs b = a;
+    b.cubic = grid_cubic(a);
     if (a.q_format == SARBP_Q_C16)
         hipLaunchKernelGGL((accumulate_kernel<QPairF16, PPT, GRID>), grid, dim3(kBpThreads), 0,
-                           stream, a);
+                           stream, b);
     else
         hipLaunchKernelGGL((accumulate_kernel<QPairF32, PPT, GRID>), grid, dim3(kBpThreads), 0,
-                           stream, a);
+                           stream, b);
     return hipGetLastError();
 }
 

@@ -350,6 +350,9 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
         a.y0 = d.y0;
         a.width = d.width;
         a.height = d.height;
+        a.full_w = d.width;
+        a.full_h = d.height;
+        a.cubic = 0;
         a.out = (double2 *)out_dev;
         a.g = p->grid;
         e = launch_accumulate(a, s);
@@ -431,6 +434,9 @@ int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int6
     a.y0 = d.y0 + row0;
     a.width = d.width;
     a.height = nrows;
+    a.full_w = d.width;
+    a.full_h = d.height;
+    a.cubic = 0;
     a.out = (double2 *)out_rows;
     a.g = p->grid;
     hipError_t e = launch_accumulate(a, stream);

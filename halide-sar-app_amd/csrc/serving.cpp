@@ -381,6 +381,8 @@ int prepare_job(sarbp_server *s, Job &j, hipStream_t st) {
     a.y0 = 0;
     a.width = d.nu;
     a.height = d.nv;
+    a.full_w = d.nu;
+    a.full_h = d.nv;
     a.out = (double2 *)(b + j.ar.out);
     a.ppt = j.ppt;
     a.g = grid;
@@ -526,6 +528,7 @@ void run_fused(sarbp_server *s, FuseSlot &slot, std::vector<Job *> &group) {
     for (size_t k = 0; k < group.size(); ++k) {
         const Job *j = group[k];
         args[k] = j->args;
+        args[k].cubic = grid_cubic(args[k]);
         for (int by = 0; by < j->d.nv; by += th)
             for (int bx = 0; bx < j->d.nu; bx += 16) tiles[t++] = {(int)k, bx, by, 0};
     }

@@ -229,6 +229,50 @@ def test_c2_full_size_sampled_and_scaling(sar, orc, scenes, pipe, torch_dev):
     assert torch.equal(out2, out * 2)
 
 
+def test_grid_expansion_forms_agree(sar, orc, scenes, pipe, torch_dev):
+    """Grid mode on the C2 scene (2048^2 x 2000 pulses, tiles of 16 x 64): the whole image in one
+    launch (4096 tiles, at least two device-fulls on a 256-CU device: the expansion evaluated as a
+    cubic in the row offset) against the same image in four launches of 512 rows (1024 tiles each:
+    the p, q form of the same polynomial). Equal tiles give equal expansion centres, so the images
+    differ by rounding only (~1e-9); a wrong coefficient of the cubic costs whole radians of phase.
+    Then the whole image against the oracle on sampled pixels."""
+    torch = torch_dev
+    sc = scenes.config_scene("C2")
+    grid = pipe.scene_grid(sc, with_pixel_locs=False)
+    ds = pipe.DeviceScene(sc, grid)
+    assert ds.grid_mode
+    nu, nv = grid["nu"], grid["nv"]
+    whole = ds.output()
+    ds.run(ds.plan(tile_ppt=4), whole)
+    bands = ds.output()
+    for y0 in range(0, nv, 512):
+        region = (0, y0, nu, 512)
+        part = ds.output(region)
+        ds.run(ds.plan(region=region, tile_ppt=4), part)
+        bands[y0:y0 + 512] = part
+    torch.cuda.synchronize()
+    a, b = whole.cpu().numpy(), bands.cpu().numpy()
+    assert np.all(np.isfinite(a))
+    e = rel_l2(a, b)
+    print(f"C2 grid mode, one launch vs four bands: rel {e:.2e}")
+    assert e < 1e-7
+    rng = np.random.default_rng(17)
+    xs = np.concatenate([np.arange(0, nu, 8), [0, nu - 1, 0, nu - 1], rng.integers(0, nu, 512)])
+    ys = np.concatenate([np.full(nu // 8, nv // 2 + 5), [0, 0, nv - 1, nv - 1],
+                         rng.integers(0, nv, 512)])
+    phs = ds.phs.cpu().numpy()
+    Q = orc.range_profiles(phs, sc.k_r, sc.taylor, ds.n_fft)
+    _, nr0 = orc.norm_r0(sc.pos)
+    pts = sar.grid_points(grid, xs, ys)
+    cnt = len(xs)
+    ref = orc.backproject_points(Q, sc.nsamples, sc.delta_r, sc.k_r[sc.nsamples // 2], sc.pos,
+                                 nr0, pts, cnt, 1, np.arange(cnt, dtype=np.int32),
+                                 np.zeros(cnt, np.int32), nthreads=NTHREADS)
+    e = rel_l2(a[ys, xs], ref)
+    print(f"C2 grid mode: sampled rel_l2 {e:.3e} over {cnt} pixels")
+    assert e < TOL
+
+
 def test_c3_full_size_sampled(sar, orc, scenes, pipe, torch_dev):
     """Configuration C3 at full size (8192^2 x 8000 pulses x 4096 samples, N_fft 16384: the largest
     FFT held in LDS), pixel locations generated on the device: oracle on 1536 sampled pixels

@@ -229,13 +229,15 @@ def test_c2_full_size_sampled_and_scaling(sar, orc, scenes, pipe, torch_dev):
     assert torch.equal(out2, out * 2)
 
 
-def test_grid_expansion_forms_agree(sar, orc, scenes, pipe, torch_dev):
-    """Grid mode on the C2 scene (2048^2 x 2000 pulses, tiles of 16 x 64): the whole image in one
-    launch (4096 tiles, at least two device-fulls on a 256-CU device: the expansion evaluated as a
-    cubic in the row offset) against the same image in four launches of 512 rows (1024 tiles each:
-    the p, q form of the same polynomial). Equal tiles give equal expansion centres, so the images
-    differ by rounding only (~1e-9); a wrong coefficient of the cubic costs whole radians of phase.
-    Then the whole image against the oracle on sampled pixels."""
+@pytest.mark.parametrize("ppt", [2, 3, 4])
+def test_grid_expansion_forms_agree(sar, orc, scenes, pipe, torch_dev, ppt):
+    """Grid mode on the C2 scene (2048^2 x 2000 pulses) at tiles of 16 x 16 ppt: the whole image in
+    one launch (8192, 5504 or 4096 tiles, at least two device-fulls on a 256-CU device: the
+    expansion evaluated as a cubic in the row offset, in blocks of 16 pulses at ppt 2 and 32 above)
+    against the same image in bands of 8 tile rows (1024 tiles each: the p, q form of the same
+    polynomial). Bands start on tile rows, so the tiles and their expansion centres are the same
+    and the images differ by rounding only (~1e-8); a wrong coefficient of the cubic costs whole
+    radians of phase. Then the whole image against the oracle on sampled pixels."""
     torch = torch_dev
     sc = scenes.config_scene("C2")
     grid = pipe.scene_grid(sc, with_pixel_locs=False)
@@ -243,18 +245,20 @@ def test_grid_expansion_forms_agree(sar, orc, scenes, pipe, torch_dev):
     assert ds.grid_mode
     nu, nv = grid["nu"], grid["nv"]
     whole = ds.output()
-    ds.run(ds.plan(tile_ppt=4), whole)
+    ds.run(ds.plan(tile_ppt=ppt), whole)
     bands = ds.output()
-    for y0 in range(0, nv, 512):
-        region = (0, y0, nu, 512)
+    band = 8 * 16 * ppt
+    for y0 in range(0, nv, band):
+        h = min(band, nv - y0)
+        region = (0, y0, nu, h)
         part = ds.output(region)
-        ds.run(ds.plan(region=region, tile_ppt=4), part)
-        bands[y0:y0 + 512] = part
+        ds.run(ds.plan(region=region, tile_ppt=ppt), part)
+        bands[y0:y0 + h] = part
     torch.cuda.synchronize()
     a, b = whole.cpu().numpy(), bands.cpu().numpy()
     assert np.all(np.isfinite(a))
     e = rel_l2(a, b)
-    print(f"C2 grid mode, one launch vs four bands: rel {e:.2e}")
+    print(f"C2 grid mode, ppt {ppt}, one launch vs bands of {band} rows: rel {e:.2e}")
     assert e < 1e-7
     rng = np.random.default_rng(17)
     xs = np.concatenate([np.arange(0, nu, 8), [0, nu - 1, 0, nu - 1], rng.integers(0, nu, 512)])

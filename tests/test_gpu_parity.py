@@ -277,6 +277,50 @@ def test_grid_expansion_forms_agree(sar, orc, scenes, pipe, torch_dev, ppt):
     assert e < TOL
 
 
+def test_grid_cubic_sheared_axes(sar, orc, scenes, pipe, torch_dev):
+    """The C2 scene on a sheared grid (v_hat turned 12 degrees towards u_hat, u_hat.v_hat = 0.21):
+    the cross term sigma u_hat.v_hat of the cubic's coefficients, zero on an orthogonal plane, is
+    then live. One launch (cubic form) against bands of 8 tile rows (p, q form) and against the
+    oracle on sampled pixels of the same grid."""
+    torch = torch_dev
+    sc = scenes.config_scene("C2")
+    base = pipe.scene_grid(sc, with_pixel_locs=False)
+    uh, vh = np.asarray(base["u_hat"], float), np.asarray(base["v_hat"], float)
+    ang = np.deg2rad(12.0)
+    vs = np.cos(ang) * vh + np.sin(ang) * uh
+    nu, nv = base["nu"], base["nv"]
+    grid = sar.image_grid(nu, nv, base["d_u"], base["d_v"], uh, vs, with_pixel_locs=False)
+    ds = pipe.DeviceScene(sc, grid)
+    assert ds.grid_mode and abs(float(uh @ vs)) > 0.2
+    whole = ds.output()
+    ds.run(ds.plan(tile_ppt=4), whole)
+    bands = ds.output()
+    for y0 in range(0, nv, 512):
+        region = (0, y0, nu, 512)
+        part = ds.output(region)
+        ds.run(ds.plan(region=region, tile_ppt=4), part)
+        bands[y0:y0 + 512] = part
+    torch.cuda.synchronize()
+    a, b = whole.cpu().numpy(), bands.cpu().numpy()
+    e = rel_l2(a, b)
+    print(f"C2 sheared grid, one launch vs bands: rel {e:.2e}")
+    assert np.all(np.isfinite(a)) and e < 1e-7
+    rng = np.random.default_rng(23)
+    xs = np.concatenate([[0, nu - 1, 0, nu - 1], rng.integers(0, nu, 768)])
+    ys = np.concatenate([[0, 0, nv - 1, nv - 1], rng.integers(0, nv, 768)])
+    phs = ds.phs.cpu().numpy()
+    Q = orc.range_profiles(phs, sc.k_r, sc.taylor, ds.n_fft)
+    _, nr0 = orc.norm_r0(sc.pos)
+    pts = sar.grid_points(grid, xs, ys)
+    cnt = len(xs)
+    ref = orc.backproject_points(Q, sc.nsamples, sc.delta_r, sc.k_r[sc.nsamples // 2], sc.pos,
+                                 nr0, pts, cnt, 1, np.arange(cnt, dtype=np.int32),
+                                 np.zeros(cnt, np.int32), nthreads=NTHREADS)
+    e = rel_l2(a[ys, xs], ref)
+    print(f"C2 sheared grid: sampled rel_l2 {e:.3e} over {cnt} pixels")
+    assert e < TOL
+
+
 def test_c3_full_size_sampled(sar, orc, scenes, pipe, torch_dev):
     """Configuration C3 at full size (8192^2 x 8000 pulses x 4096 samples, N_fft 16384: the largest
     FFT held in LDS), pixel locations generated on the device: oracle on 1536 sampled pixels

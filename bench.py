@@ -7,8 +7,9 @@ samples (N_fft 8192, Taylor 30, linear spotlight path at 15 km). Prints one JSON
 
   value     = pixel-pulses / step time (Gpixel*pulses/s)
   roofline  = the accumulation kernel: algorithmic FLOP (30 per pixel-pulse) / its mean duration,
-              against the FP32 vector peak; HBM traffic per launch from the committed rocprofv3
-              counter summary when present
+              against the FP32 vector peak; HBM traffic per launch and the binding resource (the
+              texture data path, with its busy fraction) from the committed rocprofv3 counter
+              summary when present, with the gathers' bytes per second measured live
   rel_l2    = relative L2 error of the timed image vs the CPU FP64 oracle on sampled pixels
   cpu_baseline = the oracle (a C restatement of the reference CPU schedule's loop nest) timed on
               the host cores on a bounded row sample, extrapolated to the whole image
@@ -218,13 +219,18 @@ def main():
     ppt_used = plan.tile_ppt
     kernel = (f"accumulate_kernel<{'QPairF16' if qf == sar.Q_C16 else 'QPairF32'}, {ppt_used}, "
               f"{'grid' if ds.grid_mode else 'pixel_locs'}>")
-    traffic = None
+    traffic = limiter = None
     pmc = os.path.join(ROOT, "profiles", f"accumulate_{args.config}_hbm_traffic.json")
     if os.path.exists(pmc):
         with open(pmc) as f:
             rec = json.load(f)
         if rec.get("kernel") == kernel:   # counters of this kernel on this workload only
             traffic = rec.get("bytes_per_launch")
+            limiter = rec.get("limiter")
+    if limiter is not None:
+        # the gathers' bytes through the texture path, live: one 16-byte row element per pixel-pulse
+        limiter = dict(limiter, gather_tb_per_s=round(
+            pp * limiter["gather_bytes_per_pixel_pulse"] / (acc_ms * 1e-3) / 1e12, 3))
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -251,7 +257,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": traffic, "kernel": kernel,
-                     "flop_per_launch": flop},
+                     "flop_per_launch": flop, "limiter": limiter},
         "cpu_baseline": cpu,
         "abi_call": abi,
     }

@@ -365,10 +365,12 @@ class Plan:
         if rv != 0:
             raise RuntimeError(f"sarbp_plan_create failed ({rv}): {last_error()}")
 
-    def __del__(self):
+    def __del__(self, _destroy=_lib.sarbp_plan_destroy):
+        # the library function is bound at definition: at interpreter exit the module's globals may
+        # already be cleared when the last plans are collected
         p = getattr(self, "_p", None)
         if p is not None and p.value:
-            _lib.sarbp_plan_destroy(p)
+            _destroy(p)
             self._p = None
 
     def set_grid(self, d_u, d_v, u_hat, v_hat):

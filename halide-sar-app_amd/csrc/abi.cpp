@@ -18,6 +18,7 @@
 #include <tuple>
 #include <vector>
 
+#include "abi_entry.h"
 #include "internal.h"
 
 using namespace sarbp;
@@ -458,6 +459,14 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
 
 }  // namespace
 
+namespace sarbp {
+int abi_backprojection(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t taylor_s_l,
+                       int32_t N_fft, double delta_r, halide_buffer_t *u, halide_buffer_t *v,
+                       halide_buffer_t *pos, halide_buffer_t *r, halide_buffer_t *output_img) {
+    return backprojection_impl(phs, k_r, taylor_s_l, N_fft, delta_r, u, v, pos, r, output_img);
+}
+}  // namespace sarbp
+
 extern "C" int backprojection(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t taylor_s_l,
                               int32_t N_fft, double delta_r, halide_buffer_t *u, halide_buffer_t *v,
                               halide_buffer_t *pos, halide_buffer_t *r,
@@ -625,8 +634,9 @@ extern "C" int img_output_u8(halide_buffer_t *dB, double dB_min, double dB_max,
     return 0;
 }
 
+namespace sarbp {
 // dft.cpp:20-30 builds the process-wide FFTW plan; here it builds the device twiddle table.
-extern "C" void dft_init_fftw(size_t N_fft) {
+void abi_dft_init(size_t N_fft) {
     if (N_fft < 2 || N_fft > (size_t)kMaxFft || (N_fft & (N_fft - 1))) return;
     if (sarbp_device_count() <= 0) return;
     int dev = 0;
@@ -636,4 +646,9 @@ extern "C" void dft_init_fftw(size_t N_fft) {
 
 // dft.cpp:32-35 destroys the plan. Twiddle tables are shared by every plan of the process and are
 // kept until it exits, so that calls in flight on other threads are unaffected.
-extern "C" void dft_destroy_fftw(void) {}
+void abi_dft_destroy() {}
+}  // namespace sarbp
+
+extern "C" void dft_init_fftw(size_t N_fft) { abi_dft_init(N_fft); }
+
+extern "C" void dft_destroy_fftw(void) { abi_dft_destroy(); }

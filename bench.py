@@ -58,8 +58,9 @@ def main():
                          "(reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--abi-call", type=int, default=1,
                     help="also time one backprojection() call with host buffers (1) or not (0)")
-    ap.add_argument("--serving", default="C4", choices=["none", "C4", "C5"],
-                    help="serving job stream whose latency/throughput figures the line carries")
+    ap.add_argument("--serving", default="C4,C5",
+                    help="comma-separated serving job streams (C4, C5) whose latency/throughput "
+                         "figures the line carries, or none")
     ap.add_argument("--serving-jobs", type=int, default=0,
                     help="cut the serving stream to its first N jobs (0: the whole stream)")
     ap.add_argument("--serving-load", type=float, default=0.7,
@@ -123,20 +124,28 @@ def main():
     npix = grid["nu"] * grid["nv"]
     dB = torch.empty(npix, dtype=torch.float64, device=ds.dev)
     u8 = torch.empty(npix, dtype=torch.uint8, device=ds.dev)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # one pass (max |z| found by the accumulation: sarbp_plan_img_output), and the two-pass form
+    # for an arbitrary image (sarbp_img_output_device)
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    plan.img_output(out, dB, u8, sc.db_min, sc.db_max)
     sar.img_output_device(out, npix, dB, u8, sc.db_min, sc.db_max)
     e0.record(stream)
     for _ in range(5):
-        sar.img_output_device(out, npix, dB, u8, sc.db_min, sc.db_max)
+        plan.img_output(out, dB, u8, sc.db_min, sc.db_max)
     e1.record(stream)
+    for _ in range(5):
+        sar.img_output_device(out, npix, dB, u8, sc.db_min, sc.db_max)
+    e2.record(stream)
     torch.cuda.synchronize()
     img_ms = e0.elapsed_time(e1) / 5
+    img2_ms = e1.elapsed_time(e2) / 5
 
     # accuracy of the timed image vs the oracle, and the CPU baseline on a row sample
     import oracle
     img = out.cpu().numpy()
     phs = ds.phs.cpu().numpy()
-    nthreads = max(1, min(16, os.cpu_count() or 1))
+    cpu_info = host_cpu()
+    nthreads = cpu_info["threads"]
     t_q0 = time.perf_counter()
     Q = oracle.range_profiles(phs, sc.k_r, sc.taylor, ds.n_fft)
     _, nr0 = oracle.norm_r0(sc.pos)
@@ -167,20 +176,23 @@ def main():
         call = (phs, sc.k_r, sc.taylor, ds.n_fft, sc.delta_r, band["u"], band["v"], sc.pos,
                 band["pixel_locs"], out_h)
         # two warm-up calls (plan creation, staging allocation, first touch of host pages), then
-        # the median of three
+        # the median of five
         rv = sar.backprojection(*call) or sar.backprojection(*call)
         times = []
-        for _ in range(3):
+        for _ in range(5):
             t_a = time.perf_counter()
             rv = rv or sar.backprojection(*call)
             times.append((time.perf_counter() - t_a) * 1e3)
         if rv != 0:
             raise SystemExit(f"backprojection() failed: {rv} {sar.last_error()}")
-        abi_ms = sorted(times)[1]
-        abi = {"ms": round(abi_ms, 3), "gpixel_pulses_per_s": round(pp / abi_ms / 1e6, 3),
+        abi_ms = sorted(times)[2]
+        abi = {"what": ("one backprojection() call with host buffers, as sarbp.cpp:276-280 times "
+                        "it: PCIe copies of phs, pixel_locs (24 B/pixel) and the f64 image "
+                        "included; median of 5"),
+               "ms": round(abi_ms, 3), "gpixel_pulses_per_s": round(pp / abi_ms / 1e6, 3),
                "runs_ms": [round(t, 3) for t in times],
                "rel_l2_vs_resident": float(np.linalg.norm(out_h - img) / np.linalg.norm(img))}
-        log(f"bench: backprojection() with host buffers {abi_ms:.2f} ms (median of 3)")
+        log(f"bench: backprojection() with host buffers {abi_ms:.2f} ms (median of 5)")
         del out_h
 
     cpu = None
@@ -203,6 +215,11 @@ def main():
         t_full1 = t_q + t_rows1 * grid["nv"] / rows1
         cpu = {"value": round(pp / t_full / 1e9, 5), "unit": "Gpixel·pulses/s",
                "cores": nthreads, "kind": "port",
+               "code": ("oracle/sarbp_oracle.c: the reference CPU schedule's loop nest "
+                        "(backprojection.cpp:302-314: rows serial, 4-pixel groups over threads, "
+                        "norm_rr0[4 x np] staged per group) in scalar f64 C with libm sin/cos; "
+                        "not vectorised"),
+               "usable_cores": cpu_info["usable"], "cpu_model": cpu_info["model"],
                "sample": (f"{rows} of {grid['nv']} rows ({rows * grid['nu']} pixels) x "
                           f"{sc.npulses} pulses, median of 3 runs {t_rows:.2f} s "
                           f"(runs {runs[0]:.2f}/{runs[1]:.2f}/{runs[2]:.2f} s) + all "
@@ -219,18 +236,28 @@ def main():
     ppt_used = plan.tile_ppt
     kernel = (f"accumulate_kernel<{'QPairF16' if qf == sar.Q_C16 else 'QPairF32'}, {ppt_used}, "
               f"{'grid' if ds.grid_mode else 'pixel_locs'}>")
+    # HBM traffic per launch and the binding resource are not measured by this run: they come from
+    # the committed rocprofv3 counter summary of the same kernel on the same workload (the line
+    # names the file); nothing is reported when the summary covers another kernel.
     traffic = limiter = None
-    pmc = os.path.join(ROOT, "profiles", f"accumulate_{args.config}_hbm_traffic.json")
+    traffic_source = None
+    pmc_rel = os.path.join("profiles", f"accumulate_{args.config}_hbm_traffic.json")
+    pmc = os.path.join(ROOT, pmc_rel)
     if os.path.exists(pmc):
         with open(pmc) as f:
             rec = json.load(f)
         if rec.get("kernel") == kernel:   # counters of this kernel on this workload only
             traffic = rec.get("bytes_per_launch")
             limiter = rec.get("limiter")
+            traffic_source = (f"{pmc_rel}: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (and the "
+                              f"limiter's counters) of this kernel on this workload, scaled by a "
+                              f"copy kernel; not measured by this run")
     if limiter is not None:
-        # the gathers' bytes through the texture path, live: one 16-byte row element per pixel-pulse
-        limiter = dict(limiter, gather_tb_per_s=round(
-            pp * limiter["gather_bytes_per_pixel_pulse"] / (acc_ms * 1e-3) / 1e12, 3))
+        limiter = dict(limiter, source=pmc_rel)
+        if "gather_bytes_per_pixel_pulse" in limiter:
+            # the gathers' bytes through the texture path at this run's rate
+            limiter["gather_tb_per_s"] = round(
+                pp * limiter["gather_bytes_per_pixel_pulse"] / (acc_ms * 1e-3) / 1e12, 3)
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -253,32 +280,60 @@ def main():
         },
         "rel_l2": rel,
         "stage_ms": {"range_profiles": round(fft_ms, 4), "accumulate": round(acc_ms, 4),
-                     "img_output_dB_u8": round(img_ms, 4)},
+                     "img_output_dB_u8": round(img_ms, 4),
+                     "img_output_dB_u8_two_pass": round(img2_ms, 4)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                     "traffic": traffic, "kernel": kernel,
-                     "flop_per_launch": flop, "limiter": limiter},
+                     "traffic": traffic, "traffic_source": traffic_source, "kernel": kernel,
+                     "flop_per_launch": flop,
+                     "achieved_from": (f"{pp:.4g} pixel-pulses x 30 FLOP / mean accumulation "
+                                       f"duration {acc_ms:.4f} ms (HIP events on the launch "
+                                       f"stream, {args.steps} steps)"),
+                     "measured_limiter": limiter},
         "cpu_baseline": cpu,
         "abi_call": abi,
     }
-    if args.serving != "none":
-        line["serving"] = run_serving(args)
+    streams = [c for c in args.serving.split(",") if c and c != "none"]
+    if streams:
+        line["serving"] = run_serving(args, streams[0])
+        for extra in streams[1:]:
+            line[f"serving_{extra}"] = run_serving(args, extra)
     print(json.dumps(line), flush=True)
 
 
-def run_serving(args) -> dict:
+def host_cpu() -> dict:
+    """The host CPU: model name, the cores this process may run on (affinity mask) and the threads
+    the CPU baseline uses: all of them, or the CPU share OMP_NUM_THREADS grants when it is set."""
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(usable, int(share)) if share.isdigit() and int(share) > 0 else usable
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "usable": usable, "threads": max(1, threads)}
+
+
+def run_serving(args, config) -> dict:
     """Serving figures: p50/p99 latency per job and jobs/s of the continuous-batching engine and of
     the one-job-per-launch configuration, both at the same Poisson arrival rates (`stream`: 70 % of
     the engine's capacity; `stream_low`: 70 % of the one-job-per-launch capacity)."""
     sv = importlib.import_module("halide-sar-app_amd.serving")
-    specs = sv.c4_specs() if args.serving == "C4" else sv.c5_specs()
+    if config not in ("C4", "C5"):
+        raise SystemExit(f"bench.py: unknown serving stream {config!r} (C4, C5 or none)")
+    specs = sv.c4_specs() if config == "C4" else sv.c5_specs()
     if args.serving_jobs:
         specs = specs[:args.serving_jobs]
     res = sv.benchmark(specs, load=args.serving_load, log=log)
 
     def rnd(d):
         return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
-    return {"config": args.serving, "jobs": len(specs), "load": args.serving_load,
+    return {"config": config, "jobs": len(specs), "load": args.serving_load,
             "arrival_rate_hz": round(res["arrival_rate_hz"], 2),
             "arrival_rate_low_hz": round(res["arrival_rate_low_hz"], 2),
             "engine": {k: rnd(v) for k, v in res["serving"].items()},

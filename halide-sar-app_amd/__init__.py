@@ -66,8 +66,10 @@ ERR = dict(success=0, generic_error=-1, bad_type=-3, access_out_of_bounds=-4,
 
 NORM_R0_F32_STRICT, NORM_R0_F32_FMA, NORM_R0_F64 = 0, 1, 2
 TAYLOR_DEN_INT, TAYLOR_DEN_F32 = 0, 1
+POLICY_DEFAULT = -1   # serving jobs: the process default at submission
 Q_C32, Q_C16 = 0, 1
 RANGE_F64, RANGE_DF32 = 0, 1
+GRID_FORM_NONE, GRID_FORM_PQ, GRID_FORM_CUBIC = 0, 1, 2
 
 
 class Buffer:
@@ -169,6 +171,9 @@ _lib.sarbp_plan_q_bytes.argtypes = [_P]
 _lib.sarbp_plan_q_bytes.restype = ctypes.c_size_t
 _lib.sarbp_plan_tile_ppt.argtypes = [_P]
 _lib.sarbp_plan_tile_ppt.restype = _i32
+_lib.sarbp_plan_grid_form.argtypes = [_P]
+_lib.sarbp_plan_grid_form.restype = _i32
+_lib.sarbp_plan_img_output.argtypes = [_P, _P, _P, _P, _f64, _f64, _P]
 _lib.sarbp_plan_read_pulse_norms.argtypes = [_P, _P, _i32]
 _lib.sarbp_plan_read_profiles.argtypes = [_P, _P]
 _lib.sarbp_img_output_device.argtypes = [_P, _i64, _P, _P, _f64, _f64, _P]
@@ -434,6 +439,18 @@ class Plan:
     def tile_ppt(self) -> int:
         """Pixels per thread of the accumulation tiles (the default resolved for the region)."""
         return int(_lib.sarbp_plan_tile_ppt(self._p))
+
+    @property
+    def grid_form(self) -> int:
+        """GRID_FORM_NONE / _PQ / _CUBIC: the expansion form of the plan's grid-mode tiles."""
+        return int(_lib.sarbp_plan_grid_form(self._p))
+
+    def img_output(self, img, dB=None, u8=None, dB_min=0.0, dB_max=1.0, stream=None):
+        """dB / u8 of the image the last accumulate wrote, in one pass (sarbp_plan_img_output)."""
+        rv = _lib.sarbp_plan_img_output(self._p, _ptr(img), _ptr(dB), _ptr(u8), float(dB_min),
+                                        float(dB_max), _stream_handle(stream))
+        if rv != 0:
+            raise RuntimeError(f"sarbp_plan_img_output failed ({rv}): {last_error()}")
 
 
 def img_output_device(img, npix: int, dB=None, u8=None, dB_min=0.0, dB_max=1.0, stream=None):

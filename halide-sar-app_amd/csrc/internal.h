@@ -71,9 +71,11 @@ int taylor_window(int num, int s_l, int den_policy, double *out);
 
 // Kernel launchers (kernels.hip). All are asynchronous on `stream`; return hipError_t.
 // grid: the pixel grid of grid mode (host memory) or NULL.
+// absmax (may be NULL): cleared by the same launch, for an accumulation that raises it.
 hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np,
                                const RowLayout &lay, int norm_policy, const PixelGrid *grid,
-                               PulseConst *pc, BpHeader *hdr, hipStream_t stream);
+                               PulseConst *pc, BpHeader *hdr, hipStream_t stream,
+                               unsigned long long *absmax = nullptr);
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
                                  const double *wp, int ns, int np, const RowLayout &lay,
                                  const float2 *tw, float2 *rot, void *q, int q_format,
@@ -98,6 +100,10 @@ struct BpArgs {
     int cubic;           // grid mode: expansion tiles take the cubic form (set by the launchers)
     double2 *out;
     PixelGrid g;  // g.on: locations from the grid, r unused
+    // When not NULL: the f64 bit pattern of max |z| over the pixels written is raised into *absmax
+    // (atomicMax; |z| >= 0 orders like its bits), so that the dB output needs no pass of its own.
+    // The caller clears it before the first launch of an image.
+    unsigned long long *absmax;
 };
 hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream);
 // Grid mode: whether a plan's or job's expansion tiles take the cubic form, from its whole region
@@ -122,6 +128,11 @@ int accumulate_resident_tiles(int device, int q_format, int ppt, bool grid);
 hipError_t launch_img_output(const double2 *img, int64_t npix, double *dB, uint8_t *u8,
                              double lo, double hi, unsigned long long *scratch_max,
                              hipStream_t stream);
+// dB / u8 from an image whose max |z| bits are already in *maxbits (e.g. from the accumulation).
+hipError_t launch_db_u8(const double2 *img, int64_t npix, const unsigned long long *maxbits,
+                        double *dB, uint8_t *u8, double lo, double hi, hipStream_t stream);
+// *p = 0 on `stream` (a kernel, not a memset, so that the step can sit in a captured graph).
+hipError_t launch_zero_u64(unsigned long long *p, hipStream_t stream);
 hipError_t launch_u8_only(const double *dB, int64_t npix, uint8_t *u8, double lo, double hi,
                           hipStream_t stream);
 hipError_t launch_pixel_locs(const double *u, int nu, const double *v, int nv,

@@ -15,6 +15,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -136,6 +137,22 @@ __device__ inline double grid_coord(double uh, double u, double vh, double v) {
     return (0.0 + uh * u) + vh * v;
 }
 
+// |z|^2 rounded as the reference rounds it (abs of a ComplexFunc: re * re + im * im, each product
+// and the sum rounded once; img_output.cpp:24-26). Every max and every dB value of this library
+// takes |z| as the correctly rounded sqrt of this, so a maximum found anywhere (the accumulation's
+// epilogue, absmax_kernel) equals the |z| the dB kernel forms for the same pixel.
+__device__ inline double abs2(double2 z) {
+#pragma clang fp contract(off)
+    return z.x * z.x + z.y * z.y;
+}
+
+// Raises *cell to the bit pattern of sqrt(m2) (m2 >= 0 orders like its bits), skipping the atomic
+// when the cell already holds at least that much.
+__device__ inline void raise_absmax(unsigned long long *cell, double m2) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(sqrt(m2));
+    if (b > __hip_atomic_load(cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(cell, b);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ per-pulse constants
@@ -144,10 +161,12 @@ __global__ void pulse_const_kernel(const float *__restrict__ pos, const float *_
                                    int ns, int np, double lsa, double lsr, double inv_lsr, int ext,
                                    int policy, double ux, double uy, double uz, double vx,
                                    double vy, double vz, PulseConst *__restrict__ pc,
-                                   BpHeader *__restrict__ hdr) {
+                                   BpHeader *__restrict__ hdr,
+                                   unsigned long long *__restrict__ absmax) {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
+        if (absmax) *absmax = 0ull;  // the image's max |z|, raised by the accumulation
         // k_c = k_r(nsamples / 2), f32 (backprojection.cpp:123); phase = -k_c dr in revolutions
         const double k_c = (double)k_r[ns / 2];
         const double gamma = k_c / (2.0 * M_PI);
@@ -686,6 +705,7 @@ __device__ inline void fill_expansion(const BpArgs &a, TileShared<GRID, PPT> &sh
 }
 
 // Grid mode, first stage: c_ij of block blk's pulses, one lane per pulse, in wave blk mod 4.
+static_assert(kBpThreads == 256, "fill_grid_cubic rotates its writer wave over four waves");
 template <int PPT>
 __device__ inline void fill_grid_cubic(const BpArgs &a, TileShared<true, PPT> &sh, int blk) {
     constexpr int B = kGridBlock<PPT>;
@@ -1149,6 +1169,7 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
     const PulseConst cm = a.pc[a.np / 2];
     const double k_c = a.hdr->k_c;
     const int txo = opaque(tx), tyo = opaque(ty);
+    double zmax2 = 0.0;  // largest |z|^2 the thread writes
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         const int lx = bx + txo, ly = by + tyo + 16 * j;
@@ -1160,8 +1181,17 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
         const double ang = k_c * (cm.nr0 - dm);
         double sn, cs;
         sincos(ang, &sn, &cs);
-        a.out[(size_t)ly * a.width + lx] =
-            make_double2(accr[j] * cs - acci[j] * sn, accr[j] * sn + acci[j] * cs);
+        const double2 z = make_double2(accr[j] * cs - acci[j] * sn, accr[j] * sn + acci[j] * cs);
+        a.out[(size_t)ly * a.width + lx] = z;
+        const double m2 = abs2(z);
+        zmax2 = m2 > zmax2 ? m2 : zmax2;
+    }
+    if (a.absmax) {  // the same for every lane of the launch
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(zmax2, off, 64);
+            zmax2 = o > zmax2 ? o : zmax2;
+        }
+        if ((threadIdx.x & 63) == 0) raise_absmax(a.absmax, zmax2);
     }
 }
 
@@ -1241,13 +1271,12 @@ __global__ __launch_bounds__(kImgThreads) void absmax_kernel(const double2 *__re
         for (int k = 0; k < 4; ++k) z[k] = img[i + k * stride];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const double a2 = z[k].x * z[k].x + z[k].y * z[k].y;
+            const double a2 = abs2(z[k]);
             m = a2 > m ? a2 : m;
         }
     }
     for (; i < n; i += stride) {
-        const double2 z = img[i];
-        const double a2 = z.x * z.x + z.y * z.y;
+        const double a2 = abs2(img[i]);
         m = a2 > m ? a2 : m;
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -1258,10 +1287,8 @@ __global__ __launch_bounds__(kImgThreads) void absmax_kernel(const double2 *__re
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < kImgThreads / 64; ++w) m = red[w] > m ? red[w] : m;
-        // most blocks' maxima are below the running one by now: skip their atomics
-        const unsigned long long b = (unsigned long long)__double_as_longlong(sqrt(m));
-        if (b > __hip_atomic_load(out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(out, b);
+        // most blocks' maxima are below the running one by now: their atomics are skipped
+        raise_absmax(out, m);
     }
 }
 
@@ -1275,16 +1302,54 @@ __device__ inline uint8_t to_u8(double dB, double lo, double hi) {
     return (s == s) ? (uint8_t)s : (uint8_t)0;
 }
 
+// 10 * log10(|z| / m), log10(x) = log(x) / log(Expr(10)) with the f32 log(10) (util.h:56-58)
+__device__ inline double to_db(double2 z, double m, double ln10f) {
+    return 10.0 * (log(sqrt(abs2(z)) / m) / ln10f);
+}
+
+// Each thread converts 4 consecutive pixels: four 16-byte loads, two 16-byte dB stores and one
+// 4-byte u8 store (single-byte stores would take a store instruction per pixel); a tail of
+// n mod 4 pixels is taken one at a time by the first threads. VEC false (dB not 16-byte or u8 not
+// 4-byte aligned): every pixel on its own.
+template <bool VEC>
 __global__ __launch_bounds__(kImgThreads) void db_u8_kernel(
     const double2 *__restrict__ img, int64_t n, const unsigned long long *__restrict__ maxbits,
     double ln10f, double *__restrict__ dB, uint8_t *__restrict__ u8, double lo, double hi) {
     const double m = __longlong_as_double((long long)*maxbits);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 z = img[i];
-        const double a = sqrt(z.x * z.x + z.y * z.y);
-        // 10 * log10(|z| / m), log10(x) = log(x) / log(Expr(10)) with the f32 log(10) (util.h:56-58)
-        const double v = 10.0 * (log(a / m) / ln10f);
+    if (!VEC) {
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * blockDim.x) {
+            const double v = to_db(img[i], m, ln10f);
+            if (dB) dB[i] = v;
+            if (u8) u8[i] = to_u8(v, lo, hi);
+        }
+        return;
+    }
+    const int64_t n4 = n / 4;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = t0; q < n4; q += stride) {
+        double2 z[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) z[k] = img[4 * q + k];
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = to_db(z[k], m, ln10f);
+        if (dB) {
+            double2 *d2 = reinterpret_cast<double2 *>(dB + 4 * q);
+            d2[0] = make_double2(v[0], v[1]);
+            d2[1] = make_double2(v[2], v[3]);
+        }
+        if (u8) {
+            const unsigned w = (unsigned)to_u8(v[0], lo, hi) | ((unsigned)to_u8(v[1], lo, hi) << 8) |
+                               ((unsigned)to_u8(v[2], lo, hi) << 16) |
+                               ((unsigned)to_u8(v[3], lo, hi) << 24);
+            *reinterpret_cast<unsigned *>(u8 + 4 * q) = w;
+        }
+    }
+    const int64_t i = 4 * n4 + t0;
+    if (i < n) {
+        const double v = to_db(img[i], m, ln10f);
         if (dB) dB[i] = v;
         if (u8) u8[i] = to_u8(v, lo, hi);
     }
@@ -1369,7 +1434,8 @@ __global__ __launch_bounds__(kSimThreads) void sim_phs_kernel(
 
 hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np,
                                const RowLayout &lay, int norm_policy, const PixelGrid *grid,
-                               PulseConst *pc, BpHeader *hdr, hipStream_t stream) {
+                               PulseConst *pc, BpHeader *hdr, hipStream_t stream,
+                               unsigned long long *absmax) {
     const int threads = 256;
     const int blocks = (np + threads - 1) / threads;
     const bool g = grid && grid->on;
@@ -1377,7 +1443,7 @@ hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int n
                        pos, k_r, ns, np, lay.lsa, lay.lsr, lay.inv_lsr, lay.ext, norm_policy,
                        g ? grid->uh[0] : 0.0, g ? grid->uh[1] : 0.0, g ? grid->uh[2] : 0.0,
                        g ? grid->vh[0] : 0.0, g ? grid->vh[1] : 0.0, g ? grid->vh[2] : 0.0, pc,
-                       hdr);
+                       hdr, absmax);
     return hipGetLastError();
 }
 
@@ -1562,19 +1628,40 @@ hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream) {
     }
 }
 
+hipError_t launch_zero_u64(unsigned long long *p, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_db_u8(const double2 *img, int64_t npix, const unsigned long long *maxbits,
+                        double *dB, uint8_t *u8, double lo, double hi, hipStream_t stream) {
+    if (npix <= 0 || (!dB && !u8)) return hipSuccess;
+    // 4 pixels per thread; the tail (npix mod 4 < 4 pixels) is taken by the first threads
+    const int64_t want = (npix / 4 + kImgThreads - 1) / kImgThreads;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, 4096));
+    const double ln10f = (double)std::log(10.0f);
+    const bool vec = (uintptr_t)dB % 16 == 0 && (uintptr_t)u8 % 4 == 0;
+    if (vec)
+        hipLaunchKernelGGL(db_u8_kernel<true>, dim3(blocks), dim3(kImgThreads), 0, stream, img,
+                           npix, maxbits, ln10f, dB, u8, lo, hi);
+    else
+        hipLaunchKernelGGL(db_u8_kernel<false>, dim3(blocks), dim3(kImgThreads), 0, stream, img,
+                           npix, maxbits, ln10f, dB, u8, lo, hi);
+    return hipGetLastError();
+}
+
 hipError_t launch_img_output(const double2 *img, int64_t npix, double *dB, uint8_t *u8,
                              double lo, double hi, unsigned long long *scratch_max,
                              hipStream_t stream) {
     if (npix <= 0) return hipSuccess;
     const int64_t want = (npix + kImgThreads - 1) / kImgThreads;
-    const int blocks = (int)(want < 4096 ? want : 4096);
+    const int blocks = (int)(want < kAbsmaxBlocks ? want : kAbsmaxBlocks);
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, stream, scratch_max);
-    hipLaunchKernelGGL(absmax_kernel, dim3(std::min(blocks, kAbsmaxBlocks)), dim3(kImgThreads), 0,
-                       stream, img, npix, scratch_max);
-    const double ln10f = (double)std::log(10.0f);
-    hipLaunchKernelGGL(db_u8_kernel, dim3(blocks), dim3(kImgThreads), 0, stream, img, npix,
-                       scratch_max, ln10f, dB, u8, lo, hi);
-    return hipGetLastError();
+    hipLaunchKernelGGL(absmax_kernel, dim3(blocks), dim3(kImgThreads), 0, stream, img, npix,
+                       scratch_max);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_db_u8(img, npix, scratch_max, dB, u8, lo, hi, stream);
 }
 
 hipError_t launch_u8_only(const double *dB, int64_t npix, uint8_t *u8, double lo, double hi,

@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "internal.h"
@@ -22,6 +24,7 @@ struct sarbp_plan {
     BpHeader *hdr = nullptr;    // launch-wide constants
     const float2 *tw = nullptr; // twiddles exp(-2 pi i t / N)
     PixelGrid grid{};           // grid.on: pixel locations generated in the accumulation kernel
+    unsigned long long *absmax = nullptr;  // max |z| (f64 bits) of the last accumulated image
 };
 
 namespace {
@@ -135,6 +138,7 @@ extern "C" int sarbp_plan_destroy(sarbp_plan *p) {
     (void)hipFree(p->rot);
     (void)hipFree(p->pc);
     (void)hipFree(p->hdr);
+    (void)hipFree(p->absmax);
     delete p;
     return 0;
 }
@@ -173,6 +177,7 @@ extern "C" int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out) {
               hipMalloc(&p->rot, sizeof(float2) * (size_t)len) == hipSuccess &&
               hipMalloc(&p->pc, sizeof(PulseConst) * (size_t)np) == hipSuccess &&
               hipMalloc(&p->hdr, sizeof(BpHeader)) == hipSuccess &&
+              hipMalloc(&p->absmax, sizeof(unsigned long long)) == hipSuccess &&
               (d->q_format != SARBP_Q_C16 ||
                hipMalloc(&p->qscale, sizeof(float) * (size_t)np) == hipSuccess);
     if (!ok) {
@@ -203,6 +208,35 @@ extern "C" size_t sarbp_plan_q_bytes(sarbp_plan *p) {
 }
 
 extern "C" int32_t sarbp_plan_tile_ppt(const sarbp_plan *p) { return p ? p->d.tile_ppt : 0; }
+
+extern "C" int32_t sarbp_plan_grid_form(const sarbp_plan *p) {
+    if (!p || !p->grid.on) return SARBP_GRID_FORM_NONE;
+    DeviceGuard g(p->d.device);
+    BpArgs a{};
+    a.q_format = p->d.q_format;
+    a.ppt = p->d.tile_ppt;
+    a.full_w = p->d.width;
+    a.full_h = p->d.height;
+    a.g = p->grid;
+    return grid_cubic(a) ? SARBP_GRID_FORM_CUBIC : SARBP_GRID_FORM_PQ;
+}
+
+extern "C" int sarbp_plan_img_output(sarbp_plan *p, const void *img_dev, double *dB_dev,
+                                     uint8_t *u8_dev, double dB_min, double dB_max, void *stream) {
+    if (!p || !img_dev) {
+        set_error("sarbp_plan_img_output: NULL argument");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    DeviceGuard g(p->d.device);
+    const int64_t npix = (int64_t)p->d.width * p->d.height;
+    hipError_t e = launch_db_u8((const double2 *)img_dev, npix, p->absmax, dB_dev, u8_dev, dB_min,
+                                dB_max, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_error("image-output launch failed: %s", hipGetErrorString(e));
+        return halide_error_code_device_run_failed;
+    }
+    return 0;
+}
 
 namespace {
 float half_to_float(uint16_t h) {
@@ -327,9 +361,10 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = launch_pulse_consts((const float *)pos_dev, (const float *)k_r_dev, d.nsamples,
                                        d.npulses, p->lay, d.norm_r0_policy,
-                                       p->grid.on ? &p->grid : nullptr, p->pc, p->hdr, s);
+                                       p->grid.on ? &p->grid : nullptr, p->pc, p->hdr, s,
+                                       p->absmax);
     if (e == hipSuccess) {
-        BpArgs a;
+        BpArgs a{};
         a.q = q_dev ? q_dev : p->q;
         a.q_format = d.q_format;
         a.qscale = d.q_format == SARBP_Q_C16 ? p->qscale : nullptr;
@@ -355,6 +390,7 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
         a.cubic = 0;
         a.out = (double2 *)out_dev;
         a.g = p->grid;
+        a.absmax = p->absmax;
         e = launch_accumulate(a, s);
     }
     if (e != hipSuccess) {
@@ -401,7 +437,8 @@ int plan_prepare(sarbp_plan *p, const void *phs_dev, const void *k_r_dev, const 
     const sarbp_plan_desc &d = p->d;
     hipError_t e = launch_pulse_consts((const float *)pos_dev, (const float *)k_r_dev, d.nsamples,
                                        d.npulses, p->lay, d.norm_r0_policy,
-                                       p->grid.on ? &p->grid : nullptr, p->pc, p->hdr, stream);
+                                       p->grid.on ? &p->grid : nullptr, p->pc, p->hdr, stream,
+                                       p->absmax);
     if (e != hipSuccess) {
         set_error("pulse-table launch failed: %s", hipGetErrorString(e));
         return halide_error_code_device_run_failed;
@@ -413,7 +450,7 @@ int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int6
                          int row0, int nrows, void *out_rows, hipStream_t stream) {
     DeviceGuard g(p->d.device);
     const sarbp_plan_desc &d = p->d;
-    BpArgs a;
+    BpArgs a{};
     a.q = p->q;
     a.q_format = d.q_format;
     a.qscale = d.q_format == SARBP_Q_C16 ? p->qscale : nullptr;
@@ -439,6 +476,7 @@ int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int6
     a.cubic = 0;
     a.out = (double2 *)out_rows;
     a.g = p->grid;
+    a.absmax = p->absmax;
     hipError_t e = launch_accumulate(a, stream);
     if (e != hipSuccess) {
         set_error("accumulation launch failed: %s", hipGetErrorString(e));
@@ -458,8 +496,10 @@ const sarbp_plan_desc &plan_desc(const sarbp_plan *p) { return p->d; }
 }  // namespace sarbp
 
 namespace {
+// One 8-byte max cell per (device, stream) that image output has run on: calls on one stream are
+// ordered by the stream, and calls on different streams never share a cell. Held until exit.
 std::mutex g_scratch_mu;
-std::vector<unsigned long long *> g_scratch;  // per-device 8-byte max cell
+std::map<std::pair<int, hipStream_t>, unsigned long long *> g_scratch;
 }  // namespace
 
 extern "C" int sarbp_img_output_device(const void *img_dev, int64_t npix, double *dB_dev,
@@ -477,13 +517,13 @@ extern "C" int sarbp_img_output_device(const void *img_dev, int64_t npix, double
     unsigned long long *cell = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_scratch_mu);
-        if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1, nullptr);
-        if (!g_scratch[dev] && hipMalloc(&g_scratch[dev], sizeof(unsigned long long)) != hipSuccess) {
-            g_scratch[dev] = nullptr;
+        unsigned long long *&slot = g_scratch[std::make_pair(dev, (hipStream_t)stream)];
+        if (!slot && hipMalloc(&slot, sizeof(unsigned long long)) != hipSuccess) {
+            slot = nullptr;
             set_error("scratch allocation failed");
             return halide_error_code_device_malloc_failed;
         }
-        cell = g_scratch[dev];
+        cell = slot;
     }
     hipError_t e = launch_img_output((const double2 *)img_dev, npix, dB_dev, u8_dev, dB_min, dB_max,
                                      cell, (hipStream_t)stream);

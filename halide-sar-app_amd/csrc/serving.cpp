@@ -291,8 +291,9 @@ int check_job(const sarbp_job *j) {
         set_error("sarbp_server_submit: image too large");
         return halide_error_code_buffer_extents_too_large;
     }
-    if (j->norm_r0_policy < 0 || j->norm_r0_policy > 2 || j->taylor_den_policy < 0 ||
-        j->taylor_den_policy > 1 || j->q_format < 0 || j->q_format > 1) {
+    if (j->norm_r0_policy < SARBP_POLICY_DEFAULT || j->norm_r0_policy > 2 ||
+        j->taylor_den_policy < SARBP_POLICY_DEFAULT || j->taylor_den_policy > 1 ||
+        j->q_format < 0 || j->q_format > 1) {
         set_error("sarbp_server_submit: invalid policy or format");
         return halide_error_code_constraint_violated;
     }
@@ -859,6 +860,12 @@ extern "C" int sarbp_server_submit(sarbp_server *s, const sarbp_job *job, uint64
     if (rv) return rv;
     auto j = std::make_shared<Job>();
     j->d = *job;
+    // SARBP_POLICY_DEFAULT: the process-wide defaults at submission, as a plan would take them
+    int def_norm, def_den;
+    sarbp_get_default_policies(&def_norm, &def_den);
+    if (j->d.norm_r0_policy == SARBP_POLICY_DEFAULT) j->d.norm_r0_policy = def_norm;
+    if (j->d.taylor_den_policy == SARBP_POLICY_DEFAULT) j->d.taylor_den_policy = def_den;
+    job = &j->d;
     j->lay = row_layout(job->nsamples, job->n_fft, job->delta_r);
     j->ar = layout_of(*job, j->lay);
     if ((int64_t)j->ar.total > s->cfg.hbm_budget_bytes) {

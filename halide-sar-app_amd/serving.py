@@ -141,7 +141,7 @@ class Job:
     u_hat: np.ndarray
     v_hat: np.ndarray
     out: np.ndarray            # complex128 (side, side), ideally page-locked
-    taylor_den_policy: int = _pkg.TAYLOR_DEN_F32
+    taylor_den_policy: int | None = None   # None: the process default, as for a plan
     q_format: int = _pkg.Q_C32
     keep: tuple = ()           # owners of the memory above (e.g. pinned torch tensors)
 
@@ -157,8 +157,9 @@ class Job:
         d.u_hat[:] = list(self.u_hat)
         d.v_hat[:] = list(self.v_hat)
         d.pixel_locs = None
-        d.norm_r0_policy = _pkg.NORM_R0_F32_STRICT
-        d.taylor_den_policy = self.taylor_den_policy
+        d.norm_r0_policy = _pkg.POLICY_DEFAULT
+        d.taylor_den_policy = (_pkg.POLICY_DEFAULT if self.taylor_den_policy is None
+                               else self.taylor_den_policy)
         d.q_format = self.q_format
         d.out = self.out.ctypes.data
         return d
@@ -181,7 +182,13 @@ class JobFactory:
     stream starts and kept in one page-locked slab, so transfers during the stream are
     asynchronous DMA. Outputs are page-locked images taken from per-side free lists."""
 
-    def __init__(self, specs, seed_base=4000, device=0, q_format=None):
+    # The streams' jobs ask for Taylor windows of 17-35 dB. Under the Int(32) reading of the
+    # window's F_m denominator (the library default) every level of 31 dB or more divides by zero
+    # and gives a non-finite image, as the reference's would; the streams therefore ask for the f32
+    # reading, whose window is finite at every level (the readings agree up to 17 dB, n_bar 2).
+    DEFAULT_TAYLOR_DEN = _pkg.TAYLOR_DEN_F32
+
+    def __init__(self, specs, seed_base=4000, device=0, q_format=None, taylor_den_policy=None):
         import torch
         pipe = importlib.import_module("halide-sar-app_amd.pipeline")
         self.specs = specs
@@ -203,6 +210,8 @@ class JobFactory:
             self.phs.append(view)
         torch.cuda.synchronize(dev)
         self.q_format = _pkg.Q_C32 if q_format is None else q_format
+        self.taylor_den_policy = (self.DEFAULT_TAYLOR_DEN if taylor_den_policy is None
+                                  else taylor_den_policy)
         self.grids = [self._grid(sc, sp["side"]) for sc, sp in zip(self.scenes, specs)]
         self.free = {}
         self.lock = threading.Lock()
@@ -231,7 +240,7 @@ class JobFactory:
         return Job(phs=self.phs[i], k_r=sc.k_r, pos=sc.pos, n_fft=_pkg.nfft(sc.nsamples, 2),
                    taylor=sp["taylor"], delta_r=sc.delta_r, side=g["side"], d_u=g["d_u"],
                    u_hat=g["u_hat"], v_hat=g["v_hat"], out=out, q_format=self.q_format,
-                   keep=(out_owner,))
+                   taylor_den_policy=self.taylor_den_policy, keep=(out_owner,))
 
     def pixel_pulses(self, i) -> float:
         return float(self.specs[i]["side"]) ** 2 * self.specs[i]["npulses"]

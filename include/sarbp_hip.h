@@ -3,9 +3,17 @@
  * Halide-generated `backprojection` pipeline (ISI-apex/halide-sar-app).
  *
  * Part 1 declares the drop-in entry points, exactly the symbols and argument meaning a caller of the
- * reference binds (sarbp.cpp:183-229,277,299,316; dft.h). Part 2 declares extensions that have no
- * counterpart in the reference: device-resident plans for callers whose data already lives in HBM,
- * and helpers used by the serving engine, the scene simulator and the host driver.
+ * reference binds (sarbp.cpp:183-229,277,299,316; dft.h), here with C linkage. The library also
+ * exports them with the reference's own linkage: `backprojection`, `backprojection_cuda` and
+ * `dft_init_fftw` / `dft_destroy_fftw` with C++ linkage (include/backprojection.h,
+ * include/backprojection_cuda.h; the reference's dft.h), `img_output_to_dB` / `img_output_u8` with
+ * C linkage (include/img_output_to_dB.h, include/img_output_u8.h), so a caller compiled against the
+ * reference's headers links unchanged. A translation unit that includes the reference's dft.h must
+ * not include this header (the two declare dft_init_fftw with different linkage); the pipeline
+ * headers above may be included in either order with this one.
+ * Part 2 declares extensions that have no counterpart in the reference: device-resident plans for
+ * callers whose data already lives in HBM, and helpers used by the serving engine, the scene
+ * simulator and the host driver.
  *
  * All entry points return 0 on success and a negative halide_error_code_t value on failure; none
  * aborts. Buffers are owned by the caller; inputs are read-only.
@@ -39,15 +47,21 @@ extern "C" {
  * A buffer with host == NULL and device == 0 is a bounds query: required input regions are written
  * into its dims and 0 is returned (the protocol of dft.cpp:49-58).
  */
+#ifndef SARBP_DECLARED_backprojection
+#define SARBP_DECLARED_backprojection 1
 int backprojection(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t taylor_s_l, int32_t N_fft,
                    double delta_r, halide_buffer_t *u, halide_buffer_t *v, halide_buffer_t *pos,
                    halide_buffer_t *r, halide_buffer_t *output_img);
+#endif
 
 /* Replaces `backprojection_cuda` (backprojection.cpp:356, selected at sarbp.cpp:198-200): the same
  * pipeline; on this library both names run on the GPU. */
+#ifndef SARBP_DECLARED_backprojection_cuda
+#define SARBP_DECLARED_backprojection_cuda 1
 int backprojection_cuda(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t taylor_s_l,
                         int32_t N_fft, double delta_r, halide_buffer_t *u, halide_buffer_t *v,
                         halide_buffer_t *pos, halide_buffer_t *r, halide_buffer_t *output_img);
+#endif
 
 /* Replaces img_output_to_dB (img_output.cpp:10-35, registered :54), called at sarbp.cpp:299:
  * out(x, y) = 10 * log10(|img(x, y)| / max|img|); img f64 {2, nu, nv}, out f64 {nu, nv}. */
@@ -75,6 +89,9 @@ enum sarbp_taylor_den_policy {
     SARBP_TAYLOR_DEN_INT = 0, /* (m+1)^2/(n+1)^2 as Int(32) division (default) */
     SARBP_TAYLOR_DEN_F32 = 1, /* the same ratio in f32 */
 };
+/* In a serving job (sarbp_job), either policy may be SARBP_POLICY_DEFAULT: the process-wide default
+ * at submission, which is what a plan or a Part 1 call made then would use. */
+#define SARBP_POLICY_DEFAULT (-1)
 enum sarbp_q_format { SARBP_Q_C32 = 0, SARBP_Q_C16 = 1 };
 enum sarbp_range_mode {
     SARBP_RANGE_F64 = 0, /* per-pixel f64 |r - pos| (default) */
@@ -156,6 +173,24 @@ size_t sarbp_plan_q_bytes(sarbp_plan *p);
 /* Pixels per thread of the plan's accumulation tiles (tile height 16 * this), the default resolved. */
 int32_t sarbp_plan_tile_ppt(const sarbp_plan *p);
 
+/* Which form of the range expansion the plan's grid-mode accumulation gives its expansion tiles
+ * (decided from the plan's whole region, its tile height and the device's resident tiles; tiles
+ * whose expansion bound fails take the direct form in every case). */
+enum sarbp_grid_form {
+    SARBP_GRID_FORM_NONE = 0,  /* no grid: pixel locations are read from r */
+    SARBP_GRID_FORM_PQ = 1,    /* p, q form: 5.25 f64 operations per pixel-pulse */
+    SARBP_GRID_FORM_CUBIC = 2  /* cubic in the row offset: 3 f64 FMAs per pixel-pulse */
+};
+int32_t sarbp_plan_grid_form(const sarbp_plan *p);
+
+/* dB and u8 output (img_output_to_dB, img_output_u8) of the image the plan's last accumulation
+ * wrote (sarbp_plan_execute / sarbp_plan_accumulate), in one pass over it: that accumulation also
+ * found max |z|. img_dev is that f64 {2, width, height} image; dB_dev (f64, width * height) and/or
+ * u8_dev may be NULL. Asynchronous on `stream`; must follow the accumulation in stream order and
+ * precede the plan's next one. The values equal sarbp_img_output_device's on the same image. */
+int sarbp_plan_img_output(sarbp_plan *p, const void *img_dev, double *dB_dev, uint8_t *u8_dev,
+                          double dB_min, double dB_max, void *stream);
+
 /* Grid mode: the plan's pixel locations come from the image grid (img_plane.cpp:77-113:
  * r(x, y) = (0 + u_hat u[x]) + v_hat v[y], u[x] = (x + floor(-nu/2)) d_u, v likewise, nu and nv from
  * the plan) and are generated inside the accumulation kernel, which then needs no r buffer (r_dev
@@ -174,8 +209,10 @@ int sarbp_plan_read_profiles(sarbp_plan *p, float *host_out);
  * under the plan's norm_r0 policy) to host memory; synchronous. */
 int sarbp_plan_read_pulse_norms(sarbp_plan *p, double *host_out, int32_t n);
 
-/* dB and u8 image output on the device for an f64 complex image of npix pixels. dB_dev and/or
- * u8_dev may be NULL. Synchronous with respect to `stream` only. */
+/* dB and u8 image output on the device for an f64 complex image of npix pixels: max |z|, then
+ * dB and u8 in a second pass. dB_dev and/or u8_dev may be NULL. Asynchronous on `stream`. The
+ * library keeps one 8-byte max cell per (device, stream) this has run on, so calls on different
+ * streams may overlap. */
 int sarbp_img_output_device(const void *img_dev, int64_t npix, double *dB_dev, uint8_t *u8_dev,
                             double dB_min, double dB_max, void *stream);
 
@@ -220,6 +257,11 @@ typedef struct sarbp_job {
     double d_u, d_v;          /* grid spacing: u = (x + floor(-nu/2)) d_u, v likewise */
     double u_hat[3], v_hat[3];
     const double *pixel_locs; /* optional host f64 {nu*nv, 3}; NULL: generated from the grid */
+    /* Policies as for a plan, or SARBP_POLICY_DEFAULT for the process default (that of plans and
+     * Part 1 calls: f32_strict and the Int(32) Taylor ratio unless changed). Under the Int(32)
+     * reading a Taylor level of 31 dB or more divides by zero in the window's F_m denominator
+     * (signal.h:40), so the window and the image are non-finite, as the reference's would be; a
+     * job stream that wants images at those levels asks for SARBP_TAYLOR_DEN_F32. */
     int32_t norm_r0_policy, taylor_den_policy, q_format;
     int32_t reserved0;
     void *out;                /* host complex f64 {2, nu, nv} */

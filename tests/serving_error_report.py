@@ -26,6 +26,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+import serving_check  # noqa: E402
 
 
 def main(argv=None):
@@ -59,39 +61,11 @@ def main(argv=None):
     log(f"serving error report: stream done, {sum(r['fused'] for r in recs)} of {n} jobs fused")
 
     def check(i):
-        sc, g, sp = fac.scenes[i], fac.grids[i], specs[i]
-        side, img = sp["side"], kept[i]
-        rng = np.random.default_rng(7000 + i)
-        ys = list(rng.integers(0, side, args.sample))
-        xs = list(rng.integers(0, side, args.sample))
-        py, px = np.unravel_index(int(np.argmax(np.abs(img))), img.shape)
-        h = args.half
-        wy = [y for y in range(py - h, py + h + 1) for x in range(px - h, px + h + 1)]
-        wx = [x for y in range(py - h, py + h + 1) for x in range(px - h, px + h + 1)]
-        inside = [k for k in range(len(wy)) if 0 <= wy[k] < side and 0 <= wx[k] < side]
-        ys += [wy[k] for k in inside]
-        xs += [wx[k] for k in inside]
-        ys, xs = np.array(ys, np.int32), np.array(xs, np.int32)
-        grid = sar.image_grid(side, side, g["d_u"], g["d_u"], g["u_hat"], g["v_hat"],
-                              with_pixel_locs=False)
-        pts = sar.grid_points(grid, xs, ys)
-        cnt = len(xs)
-        # the engine's jobs use the f32 Taylor denominator and the strict f32 ||pos||
-        Q = oracle.range_profiles(fac.phs[i], sc.k_r, sp["taylor"], sar.nfft(sc.nsamples, 2),
-                                  int_den=False)
-        _, nr0 = oracle.norm_r0(sc.pos)
-        ref = oracle.backproject_points(Q, sc.nsamples, sc.delta_r, sc.k_r[sc.nsamples // 2],
-                                        sc.pos, nr0, pts, cnt, 1, np.arange(cnt, dtype=np.int32),
-                                        np.zeros(cnt, np.int32))
-        got = img[ys, xs]
-        m = args.sample
-        rel = float(np.linalg.norm(got[:m] - ref[:m]) / np.linalg.norm(ref[:m]))
-        relw = float(np.linalg.norm(got[m:] - ref[m:]) / np.linalg.norm(ref[m:]))
-        moved = int(np.argmax(np.abs(got[m:])) != np.argmax(np.abs(ref[m:])))
-        r = recs[i]
-        return dict(job=i, side=side, pulses=sp["npulses"], samples=sp["ns"], taylor=sp["taylor"],
-                    fused=int(r["fused"]), ppt=int(r["ppt"]), rel_l2=rel, rel_l2_peak_window=relw,
-                    peak_moved=moved)
+        sp, r = specs[i], recs[i]
+        res = serving_check.check_job(sar, oracle, fac, i, kept[i], sample=args.sample,
+                                      half=args.half)
+        return dict(job=i, side=sp["side"], pulses=sp["npulses"], samples=sp["ns"],
+                    taylor=sp["taylor"], fused=int(r["fused"]), ppt=int(r["ppt"]), **res)
 
     t1 = time.perf_counter()
     lines = [None] * n

@@ -194,14 +194,18 @@ def test_range_profiles_shared_kernel_same_bits(sar, orc, scenes, torch_dev, q_f
         assert err < (1e-5 if q_format == 0 else 2e-3)
 
 
-def test_c2_full_size_sampled_and_scaling(sar, orc, scenes, pipe, torch_dev):
-    """Configuration C2 at full size (2048^2 x 2000 pulses, N 8192): oracle on 9216 sampled pixels
-    (two full rows, two full columns, 1024 random ones) plus exact 2x linearity of the whole image."""
+@pytest.mark.parametrize("den", ["int", "f32"])
+def test_c2_full_size_sampled_and_scaling(sar, orc, scenes, pipe, torch_dev, den):
+    """Configuration C2 at full size (2048^2 x 2000 pulses, N 8192, Taylor 30) under both readings
+    of the Taylor window's F_m denominator (signal.h:40: Int(32) division, the default, and f32,
+    the reading the textbook form pins): oracle on 9216 sampled pixels (two full rows, two full
+    columns, 1024 random ones) plus exact 2x linearity of the whole image."""
     torch = torch_dev
     sc = scenes.config_scene("C2")
     grid = pipe.scene_grid(sc)
     ds = pipe.DeviceScene(sc, grid)
-    plan = ds.plan()
+    int_den = den == "int"
+    plan = ds.plan(taylor_den_policy=sar.TAYLOR_DEN_INT if int_den else sar.TAYLOR_DEN_F32)
     out = ds.output()
     ds.run(plan, out)
     torch.cuda.synchronize()
@@ -214,12 +218,12 @@ def test_c2_full_size_sampled_and_scaling(sar, orc, scenes, pipe, torch_dev):
                          np.arange(nv), rng.integers(0, nv, 1024)])
     xs = np.concatenate([np.arange(nu), np.arange(nu), np.zeros(nv, int), np.full(nv, nu - 1),
                          rng.integers(0, nu, 1024)])
-    Q = orc.range_profiles(phs, sc.k_r, sc.taylor, ds.n_fft)
+    Q = orc.range_profiles(phs, sc.k_r, sc.taylor, ds.n_fft, int_den=int_den)
     _, nr0 = orc.norm_r0(sc.pos)
     ref = orc.backproject_points(Q, sc.nsamples, sc.delta_r, sc.k_r[sc.nsamples // 2], sc.pos,
                                  nr0, grid["pixel_locs"], nu, nv, xs, ys, nthreads=NTHREADS)
     e = rel_l2(img[ys, xs], ref)
-    print(f"C2 2048^2 x 2000: sampled rel_l2 {e:.3e} over {len(xs)} pixels")
+    print(f"C2 2048^2 x 2000, Taylor ratio {den}: sampled rel_l2 {e:.3e} over {len(xs)} pixels")
     assert e < TOL
     # scaling phs by 2 is exact in every f32/f64 operation of the path
     phs2 = ds.phs * 2
@@ -245,14 +249,18 @@ def test_grid_expansion_forms_agree(sar, orc, scenes, pipe, torch_dev, ppt):
     assert ds.grid_mode
     nu, nv = grid["nu"], grid["nv"]
     whole = ds.output()
-    ds.run(ds.plan(tile_ppt=ppt), whole)
+    wplan = ds.plan(tile_ppt=ppt)
+    assert wplan.grid_form == sar.GRID_FORM_CUBIC
+    ds.run(wplan, whole)
     bands = ds.output()
     band = 8 * 16 * ppt
     for y0 in range(0, nv, band):
         h = min(band, nv - y0)
         region = (0, y0, nu, h)
         part = ds.output(region)
-        ds.run(ds.plan(region=region, tile_ppt=ppt), part)
+        bplan = ds.plan(region=region, tile_ppt=ppt)
+        assert bplan.grid_form == sar.GRID_FORM_PQ
+        ds.run(bplan, part)
         bands[y0:y0 + h] = part
     torch.cuda.synchronize()
     a, b = whole.cpu().numpy(), bands.cpu().numpy()
@@ -293,12 +301,16 @@ def test_grid_cubic_sheared_axes(sar, orc, scenes, pipe, torch_dev):
     ds = pipe.DeviceScene(sc, grid)
     assert ds.grid_mode and abs(float(uh @ vs)) > 0.2
     whole = ds.output()
-    ds.run(ds.plan(tile_ppt=4), whole)
+    wplan = ds.plan(tile_ppt=4)
+    assert wplan.grid_form == sar.GRID_FORM_CUBIC
+    ds.run(wplan, whole)
     bands = ds.output()
     for y0 in range(0, nv, 512):
         region = (0, y0, nu, 512)
         part = ds.output(region)
-        ds.run(ds.plan(region=region, tile_ppt=4), part)
+        bplan = ds.plan(region=region, tile_ppt=4)
+        assert bplan.grid_form == sar.GRID_FORM_PQ
+        ds.run(bplan, part)
         bands[y0:y0 + 512] = part
     torch.cuda.synchronize()
     a, b = whole.cpu().numpy(), bands.cpu().numpy()
@@ -383,6 +395,64 @@ def test_img_output_parity(sar, orc, torch_dev):
     sb = sar.Buffer(sub, halide_dims=[(5, 12, 1), (20, 10, 12)])
     assert sar.img_output_to_dB(img, sb) == 0
     assert np.abs(sub - ref[20:30, 5:17]).max() < 1e-9
+
+
+def test_plan_img_output_one_pass(sar, orc, scenes, pipe, torch_dev):
+    """A plan's dB / u8 output (max |z| found by its accumulation, one pass over the image) equals
+    the two-pass device output of the same image bit for bit, and the oracle's dB to 1e-9; the
+    C1 image, grid mode and a pixel_locs plan."""
+    torch = torch_dev
+    sc = scenes.config_scene("C1")
+    for with_r in (False, True):
+        grid = pipe.scene_grid(sc, with_pixel_locs=with_r)
+        ds = pipe.DeviceScene(sc, grid)
+        plan = ds.plan()
+        out = ds.output()
+        ds.run(plan, out)
+        npix = ds.nu * ds.nv
+        d1 = torch.empty(npix, dtype=torch.float64, device=ds.dev)
+        u1 = torch.empty(npix, dtype=torch.uint8, device=ds.dev)
+        d2, u2 = torch.empty_like(d1), torch.empty_like(u1)
+        plan.img_output(out, d1, u1, sc.db_min, sc.db_max)
+        sar.img_output_device(out, npix, d2, u2, sc.db_min, sc.db_max)
+        torch.cuda.synchronize()
+        assert torch.equal(d1, d2) and torch.equal(u1, u2), with_r
+        img = out.cpu().numpy()
+        ref = orc.img_output_to_dB(img)
+        assert np.abs(d1.cpu().numpy().reshape(img.shape) - ref).max() < 1e-9
+        assert float(d1.max()) == 0.0   # the brightest pixel is 0 dB exactly
+        # a second image through the same plan clears and re-finds the maximum
+        half = ds.output()
+        plan.execute(ds.phs * 0.5, ds.k_r, ds.pos, ds.r, half)
+        plan.img_output(half, d1, u1, sc.db_min, sc.db_max)
+        torch.cuda.synchronize()
+        assert torch.equal(d1, d2) and torch.equal(u1, u2), with_r
+
+
+def test_img_output_streams_keep_their_own_max(sar, torch_dev):
+    """Image output on two streams at once: each call's max |z| is its own image's (the library
+    keeps one max cell per stream), so images 10^3 apart in scale give the same dB."""
+    torch = torch_dev
+    g = np.load(os.path.join(GOLDEN, "c1_64.npz"))
+    img = torch.from_numpy(np.ascontiguousarray(g["img"])).cuda()
+    big = img * 1000.0
+    npix = img.numel()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    outs = []
+    torch.cuda.synchronize()
+    for k in range(20):
+        a = torch.empty(npix, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        with torch.cuda.stream(s1):
+            sar.img_output_device(img, npix, a, None, -30.0, 0.0, stream=s1)
+        with torch.cuda.stream(s2):
+            sar.img_output_device(big, npix, b, None, -30.0, 0.0, stream=s2)
+        outs.append((a, b))
+    torch.cuda.synchronize()
+    ref = outs[0][0].cpu().numpy()
+    for a, b in outs:
+        assert np.abs(a.cpu().numpy() - ref).max() < 1e-9
+        assert np.abs(b.cpu().numpy() - ref).max() < 1e-9
 
 
 def test_q_f16_storage_error(sar, orc, pipe, torch_dev, scenes):

@@ -69,6 +69,21 @@ def _alone_grid(sar, fac, i, ppt):
     return out.cpu().numpy()
 
 
+def _plan_default_policy(sar, fac, i, ppt):
+    """The job through a grid-mode plan created with the process's default policies."""
+    import torch
+    sc, g, sp = fac.scenes[i], fac.grids[i], fac.specs[i]
+    dev = torch.device("cuda", 0)
+    plan = sar.Plan(sc.nsamples, sc.npulses, sar.nfft(sc.nsamples, 2), sp["taylor"], sc.delta_r,
+                    sp["side"], sp["side"], tile_ppt=ppt, q_format=fac.q_format)
+    plan.set_grid(g["d_u"], g["d_u"], g["u_hat"], g["v_hat"])
+    out = torch.empty((sp["side"], sp["side"]), dtype=torch.complex128, device=dev)
+    plan.execute(torch.from_numpy(np.ascontiguousarray(fac.phs[i])).to(dev),
+                 torch.from_numpy(sc.k_r).to(dev), torch.from_numpy(sc.pos).to(dev), None, out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
 def _mixed(fac, **cfg):
     sv = _serving()
     with sv.Server(fuse_max_pixel_pulses=FUSE_MAX, **cfg) as srv:
@@ -203,3 +218,22 @@ def test_row_formats_and_largest_fft(sar, q_format):
     for i in range(len(specs)):
         assert np.array_equal(kept[i], _alone_grid(sar, fac, i, recs[i]["ppt"])), i
         assert np.all(np.isfinite(kept[i])), i
+
+
+def test_default_policy_job_equals_plan(sar):
+    """A job that leaves its policies to the process (SARBP_POLICY_DEFAULT) is formed as a plan
+    created with the defaults forms it (Int(32) Taylor ratio, strict f32 ||pos||), bit for bit,
+    and differs from the f32 reading at Taylor >= 18 dB."""
+    sv = _serving()
+    assert sar.get_default_policies() == (sar.NORM_R0_F32_STRICT, sar.TAYLOR_DEN_INT)
+    specs = [dict(ns=64, npulses=31, side=48, taylor=17), dict(ns=96, npulses=120, side=100, taylor=30),
+             dict(ns=128, npulses=45, side=72, taylor=24)]
+    fac = sv.JobFactory(specs, seed_base=9700, taylor_den_policy=sar.POLICY_DEFAULT)
+    print("serving jobs with the default policies", flush=True)
+    with sv.Server(fuse_max_pixel_pulses=FUSE_MAX, autotune=False) as srv:
+        recs, _, kept = sv.replay(srv, fac, None, keep_outputs=range(len(specs)))
+    for i in range(len(specs)):
+        assert np.all(np.isfinite(kept[i])), i
+        assert np.array_equal(kept[i], _plan_default_policy(sar, fac, i, recs[i]["ppt"])), i
+    f32 = _alone_grid(sar, fac, 1, recs[1]["ppt"])
+    assert rel_l2(kept[1], f32) > 1e-3

@@ -110,6 +110,26 @@ def main():
     log(f"bench: step {step_ms:.3f} ms (range profiles {fft_ms:.3f} ms, accumulate {acc_ms:.3f} ms)"
         f" -> {value:.1f} Gpixel-pulses/s")
 
+    # the same accumulation with row windows staged in LDS (sarbp_plan_desc.row_windows = 1):
+    # the same image bit for bit; timed beside the default form, not part of the step
+    win_ms = win_paths = None
+    if ds.grid_mode and qf == sar.Q_C32:
+        wplan = ds.plan(q_format=qf, tile_ppt=args.tile_ppt, row_windows=1)
+        wout = ds.output()
+        ds.run(wplan, wout)
+        torch.cuda.synchronize()
+        same = bool(torch.equal(wout, out))
+        ew = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ew[0].record(stream)
+        for _ in range(5):
+            wplan.accumulate(ds.k_r, ds.pos, ds.r, wout)
+        ew[1].record(stream)
+        torch.cuda.synchronize()
+        win_ms = ew[0].elapsed_time(ew[1]) / 5
+        win_paths = dict(wplan.tile_paths(), same_bits_as_default=same)
+        log(f"bench: accumulate with row windows in LDS {win_ms:.3f} ms (same bits: {same})")
+        del wplan, wout
+
     if args.copy_calibration_mb > 0:
         n = args.copy_calibration_mb * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device=ds.dev)
@@ -281,7 +301,9 @@ def main():
         "rel_l2": rel,
         "stage_ms": {"range_profiles": round(fft_ms, 4), "accumulate": round(acc_ms, 4),
                      "img_output_dB_u8": round(img_ms, 4),
-                     "img_output_dB_u8_two_pass": round(img2_ms, 4)},
+                     "img_output_dB_u8_two_pass": round(img2_ms, 4),
+                     "accumulate_row_windows": round(win_ms, 4) if win_ms else None},
+        "row_windows_tiles": win_paths,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": traffic, "traffic_source": traffic_source, "kernel": kernel,

@@ -154,7 +154,7 @@ class PlanDesc(ctypes.Structure):
                 ("width", _i32), ("height", _i32), ("r_min", _i64), ("r_extent", _i64),
                 ("norm_r0_policy", _i32), ("taylor_den_policy", _i32), ("q_format", _i32),
                 ("range_mode", _i32), ("device", _i32), ("tile_ppt", _i32),
-                ("fft_blocks", _i32), ("reserved", _i32 * 5)]
+                ("fft_blocks", _i32), ("row_windows", _i32), ("reserved", _i32 * 4)]
 
 
 _lib.sarbp_plan_desc_init.argtypes = [ctypes.POINTER(PlanDesc)]
@@ -174,6 +174,7 @@ _lib.sarbp_plan_tile_ppt.restype = _i32
 _lib.sarbp_plan_grid_form.argtypes = [_P]
 _lib.sarbp_plan_grid_form.restype = _i32
 _lib.sarbp_plan_img_output.argtypes = [_P, _P, _P, _P, _f64, _f64, _P]
+_lib.sarbp_plan_tile_paths.argtypes = [_P, _P]
 _lib.sarbp_plan_read_pulse_norms.argtypes = [_P, _P, _i32]
 _lib.sarbp_plan_read_profiles.argtypes = [_P, _P]
 _lib.sarbp_img_output_device.argtypes = [_P, _i64, _P, _P, _f64, _f64, _P]
@@ -347,7 +348,8 @@ class Plan:
 
     def __init__(self, nsamples, npulses, n_fft, taylor_s_l, delta_r, nu, nv, region=None,
                  r_min=0, r_extent=None, norm_r0_policy=None, taylor_den_policy=None,
-                 q_format=Q_C32, range_mode=RANGE_F64, device=0, tile_ppt=0, fft_blocks=0):
+                 q_format=Q_C32, range_mode=RANGE_F64, device=0, tile_ppt=0, fft_blocks=0,
+                 row_windows=0):
         d = PlanDesc()
         _lib.sarbp_plan_desc_init(ctypes.byref(d))
         d.nsamples, d.npulses, d.n_fft, d.taylor_s_l = nsamples, npulses, n_fft, taylor_s_l
@@ -364,6 +366,7 @@ class Plan:
         d.q_format, d.range_mode, d.device = q_format, range_mode, device
         d.tile_ppt = tile_ppt
         d.fft_blocks = fft_blocks
+        d.row_windows = row_windows
         self.desc = d
         self._p = _P()
         rv = _lib.sarbp_plan_create(ctypes.byref(d), ctypes.byref(self._p))
@@ -444,6 +447,16 @@ class Plan:
     def grid_form(self) -> int:
         """GRID_FORM_NONE / _PQ / _CUBIC: the expansion form of the plan's grid-mode tiles."""
         return int(_lib.sarbp_plan_grid_form(self._p))
+
+    TILE_PATHS = ("exact", "direct", "pq", "cubic_gather", "cubic_window")
+
+    def tile_paths(self) -> dict:
+        """Tiles of the last accumulation per loop (sarbp_plan_tile_paths; synchronises)."""
+        out = np.zeros(len(self.TILE_PATHS), np.int64)
+        rv = _lib.sarbp_plan_tile_paths(self._p, out.ctypes.data_as(_P))
+        if rv != 0:
+            raise RuntimeError(f"sarbp_plan_tile_paths failed ({rv}): {last_error()}")
+        return dict(zip(self.TILE_PATHS, (int(x) for x in out)))
 
     def img_output(self, img, dB=None, u8=None, dB_min=0.0, dB_max=1.0, stream=None):
         """dB / u8 of the image the last accumulate wrote, in one pass (sarbp_plan_img_output)."""

@@ -71,11 +71,11 @@ int taylor_window(int num, int s_l, int den_policy, double *out);
 
 // Kernel launchers (kernels.hip). All are asynchronous on `stream`; return hipError_t.
 // grid: the pixel grid of grid mode (host memory) or NULL.
-// absmax (may be NULL): cleared by the same launch, for an accumulation that raises it.
+// absmax, paths (may be NULL): cleared by the same launch, for an accumulation that raises them.
 hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np,
                                const RowLayout &lay, int norm_policy, const PixelGrid *grid,
                                PulseConst *pc, BpHeader *hdr, hipStream_t stream,
-                               unsigned long long *absmax = nullptr);
+                               unsigned long long *absmax = nullptr, unsigned *paths = nullptr);
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
                                  const double *wp, int ns, int np, const RowLayout &lay,
                                  const float2 *tw, float2 *rot, void *q, int q_format,
@@ -104,6 +104,20 @@ struct BpArgs {
     // (atomicMax; |z| >= 0 orders like its bits), so that the dB output needs no pass of its own.
     // The caller clears it before the first launch of an image.
     unsigned long long *absmax;
+    // Grid mode, cubic form, f32 rows: tiles may read their rows from windows staged in LDS
+    // (same bits as the gathers; 0 keeps the gathers).
+    int row_windows;
+    // When not NULL: per tile one count added to paths[k], k = the tile's loop (TilePath).
+    unsigned *paths;
+};
+// The loop a tile takes (counted in BpArgs::paths).
+enum TilePath {
+    kPathExact = 0,        // pixels beyond the rows' cover: clamped index, f64 phase fraction
+    kPathDirect = 1,       // direct form (expansion bound fails)
+    kPathPQ = 2,           // range expansion, p, q form
+    kPathCubicGather = 3,  // grid mode, cubic form, row elements gathered from HBM
+    kPathCubicWindow = 4,  // grid mode, cubic form, row elements from windows in LDS
+    kNumPaths = 5
 };
 hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream);
 // Grid mode: whether a plan's or job's expansion tiles take the cubic form, from its whole region

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -162,11 +163,14 @@ __global__ void pulse_const_kernel(const float *__restrict__ pos, const float *_
                                    int policy, double ux, double uy, double uz, double vx,
                                    double vy, double vz, PulseConst *__restrict__ pc,
                                    BpHeader *__restrict__ hdr,
-                                   unsigned long long *__restrict__ absmax) {
+                                   unsigned long long *__restrict__ absmax,
+                                   unsigned *__restrict__ paths) {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
         if (absmax) *absmax = 0ull;  // the image's max |z|, raised by the accumulation
+        if (paths)
+            for (int k = 0; k < kNumPaths; ++k) paths[k] = 0u;  // tiles per loop, counted likewise
         // k_c = k_r(nsamples / 2), f32 (backprojection.cpp:123); phase = -k_c dr in revolutions
         const double k_c = (double)k_r[ns / 2];
         const double gamma = k_c / (2.0 * M_PI);
@@ -643,14 +647,16 @@ template <bool GRID, int PPT>
 struct ExpShared {
     double k[2][kExpBlock][6];  // per pulse: T, n, h, g
 };
-template <int PPT>
-struct GridCubic {
-    double c[2][kGridBlock<PPT>][10];       // per pulse: c_00 (+ kRowMagic), c_10, c_20, c_30, c_01,
-                                            // c_11, c_21, c_02, c_12, c_03
-    double2 c01[2][kGridBlock<PPT>][16];    // per pulse and column: (C0, C1)
-    double c2[2][kGridBlock<PPT>][16];      // C2
-    double c3[2][kGridBlock<PPT>];          // C3
+template <int B>
+struct GridCubicB {
+    double c[2][B][10];       // per pulse: c_00 (+ kRowMagic), c_10, c_20, c_30, c_01, c_11, c_21,
+                              // c_02, c_12, c_03
+    double2 c01[2][B][16];    // per pulse and column: (C0, C1)
+    double c2[2][B][16];      // C2
+    double c3[2][B];          // C3
 };
+template <int PPT>
+using GridCubic = GridCubicB<kGridBlock<PPT>>;
 template <int PPT>
 struct ExpShared<true, PPT> {
     union {
@@ -659,11 +665,58 @@ struct ExpShared<true, PPT> {
     };
 };
 
-template <bool GRID, int PPT>
+// Row windows in LDS (grid mode, cubic form, f32 rows). Per block of kWinBlock pulses and per
+// pulse, the W consecutive row elements [j0_p, j0_p + W) that hold every row coordinate the tile's
+// pixels take at that pulse are copied into LDS (global_load_lds: the texture path moves W elements
+// per pulse instead of one gathered element per pixel, ~1/14 of the bytes on C2), and the pixels
+// read them there. Windows are double-buffered: block b + 1's are loaded while block b's are read.
+// W is per tile: 2 H + 2 with H a bound on |t - t_c| over the tile's pixels and all pulses, from
+// the cubic's coefficients (triangle inequality over its terms); a tile whose W exceeds the slot
+// keeps the gathers. The window offset enters the cubic's constant term: c_00 - j0_p + S s, with
+// s the pulse's slot in the two buffers and S the slot stride, so the pixel's fixed-point
+// coordinate y is its LDS element index plus kRowMagic, and both c_00 and y stay in [2^20, 2^21),
+// where subtracting an integer is exact: every weight and element is the one the gathers read, and
+// the image the same bits.
+//
+// Per-block costs are kept off the VALU: the c_ij of kWinGroup pulses are built at once by one
+// wave (one pulse per lane) into a two-group ring, and each slot's window is loaded by
+// ceil(W / 64) wave instructions whose source is a scalar base plus the lane's offset.
+#ifndef SARBP_WIN_BLOCK
+#define SARBP_WIN_BLOCK 8
+#endif
+constexpr int kWinBlock = SARBP_WIN_BLOCK;
+constexpr int kWinGroup = 32;
+static_assert(kBpPulseBlock % kWinBlock == 0 && kWinGroup % kWinBlock == 0 && kWinGroup <= 64,
+              "flush periods hold whole window blocks; one wave builds a group's c_ij");
+// Slot stride S in elements (the largest W) per tile height; 0: no window path. LDS per workgroup:
+// ~11.4 KiB of coefficients + 2 kWinBlock S 16 B of windows, within 160 KiB over the workgroups a
+// CU holds (4 at 4 pixels per thread: S 96, 35.9 KiB; 5 at 3: S 80, 31.4 KiB).
+template <int PPT>
+constexpr int kWinSlot = PPT == 4 ? 96 : PPT == 3 ? 80 : 0;
+struct WinCubic {
+    double c[2][kWinGroup][10];     // c_ij ring, two groups (c_00 + kRowMagic - j0 + S s)
+    int j0[2][kWinGroup];           // window start of each pulse
+    double2 c01[2][kWinBlock][16];  // per pulse and column: (C0, C1)
+    double c2[2][kWinBlock][16];    // C2
+    double c3[2][kWinBlock];        // C3
+};
+template <int PPT>
+struct WinShared {
+    union {
+        double k[2][kExpBlock][6];
+        GridCubicB<kGridBlock<PPT>> cub;  // tiles whose window does not fit
+        struct {
+            WinCubic wc;
+            QPairF32 e[2 * kWinBlock * (kWinSlot<PPT> > 0 ? kWinSlot<PPT> : 1)];
+        } w;
+    };
+};
+
+template <bool GRID, int PPT, bool WIN = false>
 struct alignas(16) TileShared {
-    ExpShared<GRID, PPT> x;
+    typename std::conditional<WIN, WinShared<PPT>, ExpShared<GRID, PPT>>::type x;
     double rc[3];  // centre of the expansion (metres)
-    double dd_max[kBpThreads / 64], d2_min[kBpThreads / 64];
+    double dd_max[kBpThreads / 64], d2_min[kBpThreads / 64], h_max[kBpThreads / 64];
     int far[kBpThreads / 64];
 };
 
@@ -676,9 +729,8 @@ struct ExpTerms {
     double e0[PPT], e1[PPT], e2[PPT], dd[PPT];
 };
 
-template <bool GRID, int PPT>
-__device__ inline void fill_expansion(const BpArgs &a, TileShared<GRID, PPT> &sh, int blk,
-                                      int buf) {
+template <bool GRID, int PPT, class SH>
+__device__ inline void fill_expansion(const BpArgs &a, SH &sh, int blk, int buf) {
     const int i = threadIdx.x, p = blk * kExpBlock + i;
     if (i >= kExpBlock || p >= a.np) return;
     const double inv_lsr = a.hdr->inv_lsr;
@@ -704,17 +756,13 @@ __device__ inline void fill_expansion(const BpArgs &a, TileShared<GRID, PPT> &sh
     k[5] = -2.0 * h * h;
 }
 
-// Grid mode, first stage: c_ij of block blk's pulses, one lane per pulse, in wave blk mod 4.
-static_assert(kBpThreads == 256, "fill_grid_cubic rotates its writer wave over four waves");
-template <int PPT>
-__device__ inline void fill_grid_cubic(const BpArgs &a, TileShared<true, PPT> &sh, int blk) {
-    constexpr int B = kGridBlock<PPT>;
-    const int i = (int)threadIdx.x - 64 * (blk & 3), p = blk * B + i;
-    if (i < 0 || i >= B || p >= a.np) return;
+// The cubic's coefficients c_ij for pulse p about the tile centre rc (metres): k[0] = c_00 +
+// kRowMagic, then c_10, c_20, c_30, c_01, c_11, c_21, c_02, c_12, c_03 (row units per metre^(i+j)).
+__device__ inline void cubic_coeffs(const BpArgs &a, const double *rc, int p, double (&k)[10]) {
     const double inv_lsr = a.hdr->inv_lsr, sigma = inv_lsr * inv_lsr;
     const PulseConst c = a.pc[p];
-    const double Rx = (sh.rc[0] - c.px) * inv_lsr, Ry = (sh.rc[1] - c.py) * inv_lsr,
-                 Rz = (sh.rc[2] - c.pz) * inv_lsr;
+    const double Rx = (rc[0] - c.px) * inv_lsr, Ry = (rc[1] - c.py) * inv_lsr,
+                 Rz = (rc[2] - c.pz) * inv_lsr;
     const double d = sqrt(Rx * Rx + Ry * Ry + Rz * Rz);
     const double id = 1.0 / d;
     const double h = 0.5 * id, g = -2.0 * h * h;
@@ -725,7 +773,6 @@ __device__ inline void fill_grid_cubic(const BpArgs &a, TileShared<true, PPT> &s
     const double quu = sigma * (uh[0] * uh[0] + uh[1] * uh[1] + uh[2] * uh[2]) - na * na;
     const double quv = sigma * (uh[0] * vh[0] + uh[1] * vh[1] + uh[2] * vh[2]) - na * nb;
     const double qvv = sigma * (vh[0] * vh[0] + vh[1] * vh[1] + vh[2] * vh[2]) - nb * nb;
-    double *k = sh.x.cub.c[blk & 1][i];
     k[0] = (c.B - d) + kRowMagic;
     k[1] = -na;
     k[2] = -h * quu;
@@ -738,21 +785,118 @@ __device__ inline void fill_grid_cubic(const BpArgs &a, TileShared<true, PPT> &s
     k[9] = -g * nb * qvv;
 }
 
+// Bound on |t - t_c| over a tile whose offsets satisfy |du| <= U, |dv| <= V, from the cubic's
+// coefficients (each term at its largest), plus a margin for the rounding of y.
+__device__ inline double cubic_half_span(const double (&k)[10], double U, double V) {
+    const double u2 = U * U, v2 = V * V;
+    return fabs(k[1]) * U + fabs(k[2]) * u2 + fabs(k[3]) * u2 * U + fabs(k[4]) * V +
+           fabs(k[5]) * U * V + fabs(k[6]) * u2 * V + fabs(k[7]) * v2 + fabs(k[8]) * U * v2 +
+           fabs(k[9]) * v2 * V + 1e-3;
+}
+
+// Window parameters of a tile on the window path: W elements per pulse, the largest offsets U, V
+// and the row length.
+struct WinTile {
+    int W;
+    double U, V;
+    int len;
+};
+
+// Grid mode, first stage: c_ij of block blk's pulses, one lane per pulse, in wave blk mod 4.
+static_assert(kBpThreads == 256, "fill_grid_cubic rotates its writer wave over four waves");
+template <int B>
+__device__ inline void fill_grid_cubic(const BpArgs &a, const double *rc, GridCubicB<B> &cub,
+                                       int blk) {
+    const int i = (int)threadIdx.x - 64 * (blk & 3), p = blk * B + i;
+    if (i < 0 || i >= B || p >= a.np) return;
+    double k[10];
+    cubic_coeffs(a, rc, p, k);
+    double *dst = cub.c[blk & 1][i];
+#pragma unroll
+    for (int m = 0; m < 10; ++m) dst[m] = k[m];
+}
+
+// Window path, first stage: c_ij of group g's kWinGroup pulses (one lane per pulse, wave g mod 4),
+// each with its window start j0 (clamped to the row) and c_00 - j0 + S s in place of c_00, s the
+// pulse's slot: block b = 4 g + i / 8 uses buffer b mod 2.
+template <int S>
+__device__ inline void fill_win_group(const BpArgs &a, const double *rc, WinCubic &wc, int g,
+                                      const WinTile &win) {
+    const int i = (int)threadIdx.x - 64 * (g & 3), p = g * kWinGroup + i;
+    if (i < 0 || i >= kWinGroup || p >= a.np) return;
+    double k[10];
+    cubic_coeffs(a, rc, p, k);
+    const double H = cubic_half_span(k, win.U, win.V);
+    const double t_c = k[0] - kRowMagic;
+    const int j = min(max((int)floor(t_c - H), 0), win.len - win.W);
+    const int blk = p / kWinBlock, slot = (blk & 1) * kWinBlock + p % kWinBlock;
+    wc.j0[g & 1][i] = j;
+    k[0] -= (double)(j - S * slot);
+    double *dst = wc.c[g & 1][i];
+#pragma unroll
+    for (int m = 0; m < 10; ++m) dst[m] = k[m];
+}
+
 // Grid mode, second stage: C0..C3 of block blk's pulses for the tile's 16 columns
 // (du = (x - 7.5) d_u), from the c_ij of the first stage.
-template <int PPT>
-__device__ inline void fill_grid_columns(const BpArgs &a, TileShared<true, PPT> &sh, int blk) {
-    constexpr int B = kGridBlock<PPT>;
+template <int B>
+__device__ inline void fill_grid_columns(const BpArgs &a, GridCubicB<B> &cub, int blk) {
     const int n = min(B, a.np - blk * B), buf = blk & 1;
     for (int job = threadIdx.x; job < 16 * n; job += kBpThreads) {
         const int i = job >> 4, x = job & 15;
-        const double *k = sh.x.cub.c[buf][i];
+        const double *k = cub.c[buf][i];
         const double du = ((double)x - 7.5) * a.g.d_u;
         const double c0 = __builtin_fma(__builtin_fma(__builtin_fma(k[3], du, k[2]), du, k[1]), du, k[0]);
         const double c1 = __builtin_fma(__builtin_fma(k[6], du, k[5]), du, k[4]);
-        sh.x.cub.c01[buf][i][x] = make_double2(c0, c1);
-        sh.x.cub.c2[buf][i][x] = __builtin_fma(k[8], du, k[7]);
-        if (x == 0) sh.x.cub.c3[buf][i] = k[9];
+        cub.c01[buf][i][x] = make_double2(c0, c1);
+        cub.c2[buf][i][x] = __builtin_fma(k[8], du, k[7]);
+        if (x == 0) cub.c3[buf][i] = k[9];
+    }
+}
+
+// Window path, second stage: C0..C3 of block blk's pulses for the tile's 16 columns, from the
+// group ring.
+__device__ inline void fill_win_columns(const BpArgs &a, WinCubic &wc, int blk) {
+    const int n = min(kWinBlock, a.np - blk * kWinBlock), buf = blk & 1;
+    const int g = blk / (kWinGroup / kWinBlock), i0 = (blk % (kWinGroup / kWinBlock)) * kWinBlock;
+    for (int job = threadIdx.x; job < 16 * n; job += kBpThreads) {
+        const int i = job >> 4, x = job & 15;
+        const double *k = wc.c[g & 1][i0 + i];
+        const double du = ((double)x - 7.5) * a.g.d_u;
+        const double c0 = __builtin_fma(__builtin_fma(__builtin_fma(k[3], du, k[2]), du, k[1]), du, k[0]);
+        const double c1 = __builtin_fma(__builtin_fma(k[6], du, k[5]), du, k[4]);
+        wc.c01[buf][i][x] = make_double2(c0, c1);
+        wc.c2[buf][i][x] = __builtin_fma(k[8], du, k[7]);
+        if (x == 0) wc.c3[buf][i] = k[9];
+    }
+}
+
+// Window path: copies block blk's windows into its LDS buffer with global_load_lds: slot s's
+// window is elements [S s, S s + W) of the two buffers, loaded by ceil(W / 64) wave instructions
+// (one per 64 elements; each lane writes the element at its lane index past the instruction's
+// LDS address). The source of an instruction is the row element j0 + 64 c (a wave-uniform address)
+// plus the lane. Asynchronous: the caller waits for the wave's loads (vmcnt) before the barrier
+// that publishes them.
+typedef __attribute__((address_space(3))) void lds_void;
+template <int S>
+__device__ inline void load_windows(const BpArgs &a, const WinTile &win, const WinCubic &wc,
+                                    QPairF32 *buf, int blk) {
+    // every value but the lane's is uniform: loop and addresses on the scalar unit
+    const int W = __builtin_amdgcn_readfirstlane(win.W), bk = __builtin_amdgcn_readfirstlane(blk);
+    const int len = __builtin_amdgcn_readfirstlane(a.len);
+    const int n = min(kWinBlock, a.np - bk * kWinBlock);
+    const int chunks = (W + 63) >> 6;  // 1 or 2 (W <= S <= 128)
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = bk / (kWinGroup / kWinBlock), i0 = (bk % (kWinGroup / kWinBlock)) * kWinBlock;
+    const QPairF32 *q = reinterpret_cast<const QPairF32 *>(a.q);
+    // lanes that load in a slot's first and second chunk (the second holds elements 64 .. W - 1)
+    const bool in0 = lane < W, in1 = lane + 64 < W;
+    for (int m = wave; m < n * chunks; m += kBpThreads / 64) {
+        const int sl = chunks == 2 ? m >> 1 : m, c = m - sl * chunks;
+        const int j0 = __builtin_amdgcn_readfirstlane(wc.j0[g & 1][i0 + sl]);
+        const QPairF32 *row = q + (size_t)(bk * kWinBlock + sl) * len + (j0 + 64 * c);
+        QPairF32 *dst = buf + ((bk & 1) * kWinBlock + sl) * S + 64 * c;
+        if (c == 0 ? in0 : in1) __builtin_amdgcn_global_load_lds(row + lane, (lds_void *)dst, 16, 0, 0);
     }
 }
 
@@ -819,17 +963,42 @@ __device__ inline void gather_grid(const BpArgs &a, const double (&dv)[PPT], con
     gather_fixed<QT, PPT>(a, y, p, g);
 }
 
-// The fast pulse loop over the cubic (grid mode): blocks of kGridBlock pulses; in the period of
-// block b the threads build block b + 2's c_ij and block b + 1's C0..C3, then consume block b's
-// (pipelined as in accumulate_pulses; flush after every kBpPulseBlock pulses of the whole
-// sequence). Each write goes to the buffer of the block two back, whose last reader ran in an
-// earlier period: c_ij of block b + 2 replace those of b (read by the second stage in period
-// b - 1), C of b + 1 those of b - 1 (consumed in period b - 1).
-template <typename QT, int PPT>
+// The same with the pulse's row window in LDS: y = kRowMagic + (the pixel's element index in the
+// window buffers) + fract, so the high word of y is 0x41380000 + that index, and (hi << 4) + K with
+// K = (LDS address of the buffers) - 0x13800000 (mod 2^32) is the element's LDS address: one
+// shift-add per pixel in place of the index mask.
+typedef __attribute__((address_space(3))) const f32x4 lds_f32x4;
+template <int PPT>
+__device__ inline void gather_grid_win(const double (&dv)[PPT], const double2 &c01, double c2,
+                                       double c3, unsigned K, Gather<PPT> &g) {
+    double y[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j)
+        y[j] = __builtin_fma(__builtin_fma(__builtin_fma(c3, dv[j], c2), dv[j], c01.y), dv[j], c01.x);
+    unsigned addr[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const u32x2 b = __builtin_bit_cast(u32x2, y[j]);
+        addr[j] = (b.y << 4) + K;
+        g.ww[j] = (float)b.x;  // fract(t) * 2^32: the weight unit of f32 rows
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const f32x4 v = *(lds_f32x4 *)(size_t)addr[j];
+        g.e[j] = make_float4(v.x, v.y, v.z, v.w);
+    }
+}
+
+// The fast pulse loop over the cubic (grid mode), row elements gathered from HBM: blocks of B
+// pulses; in the period of block b the threads build block b + 2's c_ij and block b + 1's C0..C3,
+// then consume block b's (pipelined as in accumulate_pulses; flush after every kBpPulseBlock pulses
+// of the whole sequence). Each write goes to the buffer of the block two back, whose last reader
+// ran in an earlier period: c_ij of block b + 2 replace those of b (read by the second stage in
+// period b - 1), C of b + 1 those of b - 1 (consumed in period b - 1).
+template <typename QT, int PPT, int B>
 __device__ inline void accumulate_pulses_grid(const BpArgs &a, const double (&dv)[PPT], int col,
-                                              TileShared<true, PPT> &sh, double (&accr)[PPT],
-                                              double (&acci)[PPT]) {
-    constexpr int B = kGridBlock<PPT>;
+                                              const double *rc, GridCubicB<B> &cub,
+                                              double (&accr)[PPT], double (&acci)[PPT]) {
     const int np = a.np;
     const float kappa_f = (float)a.hdr->kappa / weight_unit<QT>();
     float sr[PPT], si[PPT];
@@ -844,17 +1013,17 @@ __device__ inline void accumulate_pulses_grid(const BpArgs &a, const double (&dv
         }
     };
     const int nblk = (np + B - 1) / B;
-    fill_grid_cubic<PPT>(a, sh, 0);
-    if (nblk > 1) fill_grid_cubic<PPT>(a, sh, 1);
+    fill_grid_cubic<B>(a, rc, cub, 0);
+    if (nblk > 1) fill_grid_cubic<B>(a, rc, cub, 1);
     __syncthreads();
-    fill_grid_columns<PPT>(a, sh, 0);
+    fill_grid_columns<B>(a, cub, 0);
     for (int b = 0; b < nblk; ++b) {
         __syncthreads();
-        if (b + 2 < nblk) fill_grid_cubic<PPT>(a, sh, b + 2);
-        if (b + 1 < nblk) fill_grid_columns<PPT>(a, sh, b + 1);
+        if (b + 2 < nblk) fill_grid_cubic<B>(a, rc, cub, b + 2);
+        if (b + 1 < nblk) fill_grid_columns<B>(a, cub, b + 1);
         const int buf = b & 1, p0 = b * B, p1 = min(np, p0 + B);
-        const double2 *c01 = &sh.x.cub.c01[buf][0][col];
-        const double *c2 = &sh.x.cub.c2[buf][0][col], *c3 = sh.x.cub.c3[buf];
+        const double2 *c01 = &cub.c01[buf][0][col];
+        const double *c2 = &cub.c2[buf][0][col], *c3 = cub.c3[buf];
         Gather<PPT> g0, g1;
         gather_grid<QT, PPT>(a, dv, c01[0], c2[0], c3[0], p0, g0);
         for (int p = p0; p + 1 < p1; p += 2) {
@@ -871,13 +1040,70 @@ __device__ inline void accumulate_pulses_grid(const BpArgs &a, const double (&dv
     flush();
 }
 
+// The same loop reading row windows from LDS (f32 rows), in blocks of kWinBlock pulses. In the
+// period of block b the threads build the c_ij of group (b + 2) / 4 when b = 2 mod 4 (one wave),
+// block b + 1's C0..C3 and block b + 1's windows, then consume block b. Group g's ring slot was last
+// read in period 4 g - 6 (its columns and windows of blocks up to 4 g - 5), and C and windows of
+// b + 1 replace those of b - 1, consumed in period b - 1. Window loads are waited for (vmcnt)
+// before the barrier that opens the period reading them.
+template <int PPT, int S>
+__device__ inline void accumulate_pulses_win(const BpArgs &a, const double (&dv)[PPT], int col,
+                                             const double *rc, WinCubic &wc, QPairF32 *wbuf,
+                                             const WinTile &win, double (&accr)[PPT],
+                                             double (&acci)[PPT]) {
+    constexpr int B = kWinBlock, BPG = kWinGroup / kWinBlock;
+    const int np = a.np;
+    const float kappa_f = (float)a.hdr->kappa / weight_unit<QPairF32>();
+    float sr[PPT], si[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) sr[j] = si[j] = 0.f;
+    auto flush = [&] {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            accr[j] += (double)sr[j];
+            acci[j] += (double)si[j];
+            sr[j] = si[j] = 0.f;
+        }
+    };
+    const unsigned K = (unsigned)(size_t)(lds_void *)wbuf - 0x13800000u;
+    const int nblk = (np + B - 1) / B;
+    fill_win_group<S>(a, rc, wc, 0, win);
+    __syncthreads();
+    fill_win_columns(a, wc, 0);
+    load_windows<S>(a, win, wc, wbuf, 0);
+    for (int b = 0; b < nblk; ++b) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's window loads have landed
+        __syncthreads();
+        if (b % BPG == BPG - 2 && (b + 2) * B < np) fill_win_group<S>(a, rc, wc, (b + 2) / BPG, win);
+        if (b + 1 < nblk) {
+            fill_win_columns(a, wc, b + 1);
+            load_windows<S>(a, win, wc, wbuf, b + 1);
+        }
+        const int buf = b & 1, p0 = b * B, p1 = min(np, p0 + B);
+        const double2 *c01 = &wc.c01[buf][0][col];
+        const double *c2 = &wc.c2[buf][0][col], *c3 = wc.c3[buf];
+        Gather<PPT> g0, g1;
+        gather_grid_win<PPT>(dv, c01[0], c2[0], c3[0], K, g0);
+        for (int p = p0; p + 1 < p1; p += 2) {
+            const int i1 = p + 1 - p0;
+            gather_grid_win<PPT>(dv, c01[16 * i1], c2[16 * i1], c3[i1], K, g1);
+            add_terms<QPairF32, PPT, true>(g0, kappa_f, sr, si);
+            const int i2 = min(p + 2, p1 - 1) - p0;
+            gather_grid_win<PPT>(dv, c01[16 * i2], c2[16 * i2], c3[i2], K, g0);
+            add_terms<QPairF32, PPT, true>(g1, kappa_f, sr, si);
+            if (((p + 2) & (kBpPulseBlock - 1)) == 0) flush();
+        }
+        if ((p1 - p0) & 1) add_terms<QPairF32, PPT, true>(g0, kappa_f, sr, si);  // pulse p1 - 1
+    }
+    flush();
+}
+
 // The fast pulse loop over the expansion in the form above: per block of kExpBlock pulses a barrier,
 // the next block's constants, then the block's pulses pipelined as in accumulate_pulses (flush after
 // every kBpPulseBlock pulses of the whole sequence).
-template <typename QT, int PPT, bool GRID>
-__device__ inline void accumulate_pulses_expanded(const BpArgs &a, const ExpTerms<PPT> &E,
-                                                  TileShared<GRID, PPT> &sh, double (&accr)[PPT],
-                                                  double (&acci)[PPT]) {
+template <typename QT, int PPT, bool GRID, class SH>
+__device__ inline void accumulate_pulses_expanded(const BpArgs &a, const ExpTerms<PPT> &E, SH &sh,
+                                                  double (&accr)[PPT], double (&acci)[PPT]) {
     const int np = a.np;
     const float kappa_f = (float)a.hdr->kappa / weight_unit<QT>();
     float sr[PPT], si[PPT];
@@ -1038,9 +1264,11 @@ __device__ inline void grid_offsets(const BpArgs &a, int bx, int by, int tx, int
     }
 }
 
-template <typename QT, int PPT, bool GRID>
+template <typename QT, int PPT, bool GRID, bool WIN = false>
 __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
-                                            TileShared<GRID, PPT> &sh, bool cubic) {
+                                            TileShared<GRID, PPT, WIN> &sh, bool cubic) {
+    static_assert(!WIN || (GRID && sizeof(QT) == sizeof(QPairF32) && kWinSlot<PPT> > 0),
+                  "row windows: grid mode, f32 rows, tile heights with a window slot");
     const double inv_lsr = a.hdr->inv_lsr;
     // Wave w holds rows 4w..4w+3 (+16 j) of the tile's 16 columns. The memory pipeline serves a
     // gather four lanes at a time and its cost grows with the cache lines each quad of lanes
@@ -1118,22 +1346,49 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
         any_far = any_far || sh.far[w] != 0;
         dd_max = fmax(dd_max, sh.dd_max[w]);
     }
-    // the bound needs the least distance from the centre to the platform
+    // the bound needs the least distance from the centre to the platform; a cubic tile of a window
+    // kernel also needs the largest half-span of its row coordinates over the pulses
     bool expand = false;
+    WinTile wt{};
+    const bool want_win = WIN && cubic;
+    if (WIN) {
+        wt.U = 7.5 * fabs(a.g.d_u);
+        wt.V = (8.0 * PPT - 0.5) * fabs(a.g.d_v);
+        wt.len = a.len;
+    }
     if (!any_far) {
-        double d2 = INFINITY;
+        double d2 = INFINITY, hm = 0.0;
         for (int p = threadIdx.x; p < a.np; p += kBpThreads) {
             const double dx = rc[0] - a.pc[p].px, dy = rc[1] - a.pc[p].py, dz = rc[2] - a.pc[p].pz;
             d2 = fmin(d2, dx * dx + dy * dy + dz * dz);
+            if (want_win) {
+                double k[10];
+                cubic_coeffs(a, rc, p, k);
+                hm = fmax(hm, cubic_half_span(k, wt.U, wt.V));
+            }
         }
-        for (int off = 32; off > 0; off >>= 1) d2 = fmin(d2, __shfl_xor(d2, off, 64));
-        if (lane == 0) sh.d2_min[wave] = d2;
+        for (int off = 32; off > 0; off >>= 1) {
+            d2 = fmin(d2, __shfl_xor(d2, off, 64));
+            hm = fmax(hm, __shfl_xor(hm, off, 64));
+        }
+        if (lane == 0) {
+            sh.d2_min[wave] = d2;
+            sh.h_max[wave] = hm;
+        }
         __syncthreads();
 #pragma unroll
-        for (int w = 0; w < kBpThreads / 64; ++w) d2 = fmin(d2, sh.d2_min[w]);
+        for (int w = 0; w < kBpThreads / 64; ++w) {
+            d2 = fmin(d2, sh.d2_min[w]);
+            hm = fmax(hm, sh.h_max[w]);
+        }
         const double dm = sqrt(inv_lsr * inv_lsr * d2);  // row units
         expand = dm > 0.0 && dd_max <= 1e-4 * dm * dm &&
                  dd_max * dd_max <= 4.0 * kExpandTol * dm * dm * dm;
+        if (want_win && hm < 0.5 * kWinSlot<PPT>) {
+            // every floor(t) of a pulse lies in [floor(t_c - H), floor(t_c + H)]: 2 H + 2 elements
+            const int W = min((int)floor(2.0 * hm) + 2, a.len);
+            wt.W = __builtin_amdgcn_readfirstlane(W <= kWinSlot<PPT> ? W : 0);
+        }
     }
 
     // The decision is the same in every lane (it comes from LDS); readfirstlane tells the compiler,
@@ -1141,6 +1396,14 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
     // opaque copy of the lane's coordinates, so that nothing computed above stays live through a
     // pulse loop that does not use it.
     const int path = __builtin_amdgcn_readfirstlane(any_far ? 0 : expand ? 1 : 2);
+    if (a.paths && threadIdx.x == 0) {
+        const int k = path == 0   ? kPathExact
+                      : path == 2 ? kPathDirect
+                      : !(GRID && cubic) ? kPathPQ
+                      : wt.W > 0  ? kPathCubicWindow
+                                  : kPathCubicGather;
+        atomicAdd(&a.paths[k], 1u);
+    }
     double accr[PPT], acci[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) accr[j] = acci[j] = 0.0;
@@ -1149,7 +1412,17 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
             double dv[PPT];
             int col;
             grid_offsets<PPT>(a, bx, by, opaque(tx), opaque(ty), dv, col);
-            accumulate_pulses_grid<QT, PPT>(a, dv, col, sh, accr, acci);
+            if constexpr (WIN) {
+                if (wt.W > 0)
+                    accumulate_pulses_win<PPT, kWinSlot<PPT>>(a, dv, col, sh.rc, sh.x.w.wc,
+                                                              sh.x.w.e, wt, accr, acci);
+                else
+                    accumulate_pulses_grid<QT, PPT, kGridBlock<PPT>>(a, dv, col, sh.rc, sh.x.cub,
+                                                                     accr, acci);
+            } else {
+                accumulate_pulses_grid<QT, PPT, kGridBlock<PPT>>(a, dv, col, sh.rc, sh.x.cub,
+                                                                 accr, acci);
+            }
         }
     } else if (path == 1) {
         ExpTerms<PPT> E;
@@ -1217,10 +1490,15 @@ constexpr int waves_per_simd(int ppt) {
 // columns, the orientation of the reference's image plane (u_hat along range). Every workgroup of
 // a launch takes the same decision, so the order stays a bijection. a.cubic (grid mode): expansion
 // tiles take the cubic form (accumulate_pulses_grid) rather than the p, q form.
-template <typename QT, int PPT, bool GRID>
-__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(waves_per_simd<QT, GRID>(PPT))))
+// WIN: cubic tiles read their rows from windows in LDS where the window fits (see WinShared).
+#ifndef SARBP_WIN_WAVES
+#define SARBP_WIN_WAVES 0
+#endif
+template <typename QT, int PPT, bool GRID, bool WIN = false>
+__global__ __launch_bounds__(kBpThreads) __attribute__((amdgpu_waves_per_eu(
+    WIN && SARBP_WIN_WAVES > 0 ? SARBP_WIN_WAVES : waves_per_simd<QT, GRID>(PPT))))
 void accumulate_kernel(BpArgs a) {
-    __shared__ TileShared<GRID, PPT> sh;
+    __shared__ TileShared<GRID, PPT, WIN> sh;
     constexpr int th = 16 * PPT;
     const int tiles_x = (a.width + 15) / 16, tiles_y = (a.height + th - 1) / th;
     bool cols = true;
@@ -1230,7 +1508,7 @@ void accumulate_kernel(BpArgs a) {
     }
     int tx, ty;
     xcd_band_tile((int)blockIdx.x, tiles_x, tiles_y, cols, tx, ty);
-    accumulate_tile_impl<QT, PPT, GRID>(a, tx * 16, ty * th, sh, a.cubic != 0);
+    accumulate_tile_impl<QT, PPT, GRID, WIN>(a, tx * 16, ty * th, sh, a.cubic != 0);
 }
 
 // Several jobs in one launch over a tile list (job by job, row-major): block b computes tile
@@ -1435,7 +1713,7 @@ __global__ __launch_bounds__(kSimThreads) void sim_phs_kernel(
 hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int np,
                                const RowLayout &lay, int norm_policy, const PixelGrid *grid,
                                PulseConst *pc, BpHeader *hdr, hipStream_t stream,
-                               unsigned long long *absmax) {
+                               unsigned long long *absmax, unsigned *paths) {
     const int threads = 256;
     const int blocks = (np + threads - 1) / threads;
     const bool g = grid && grid->on;
@@ -1443,7 +1721,7 @@ hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int n
                        pos, k_r, ns, np, lay.lsa, lay.lsr, lay.inv_lsr, lay.ext, norm_policy,
                        g ? grid->uh[0] : 0.0, g ? grid->uh[1] : 0.0, g ? grid->uh[2] : 0.0,
                        g ? grid->vh[0] : 0.0, g ? grid->vh[1] : 0.0, g ? grid->vh[2] : 0.0, pc,
-                       hdr, absmax);
+                       hdr, absmax, paths);
     return hipGetLastError();
 }
 
@@ -1533,6 +1811,14 @@ static hipError_t launch_accumulate_ppt(const BpArgs &a, hipStream_t stream) {
     const dim3 grid((unsigned)(((a.width + 15) / 16) * ((a.height + 16 * PPT - 1) / (16 * PPT))));
     BpArgs b = a;
     b.cubic = grid_cubic(a);
+    if constexpr (GRID && kWinSlot<PPT> > 0) {
+        // the window kernel gives the same bits as the gather kernel; it differs in LDS use only
+        if (b.cubic && a.q_format == SARBP_Q_C32 && a.row_windows) {
+            hipLaunchKernelGGL((accumulate_kernel<QPairF32, PPT, true, true>), grid,
+                               dim3(kBpThreads), 0, stream, b);
+            return hipGetLastError();
+        }
+    }
     if (a.q_format == SARBP_Q_C16)
         hipLaunchKernelGGL((accumulate_kernel<QPairF16, PPT, GRID>), grid, dim3(kBpThreads), 0,
                            stream, b);

@@ -25,6 +25,7 @@ struct sarbp_plan {
     const float2 *tw = nullptr; // twiddles exp(-2 pi i t / N)
     PixelGrid grid{};           // grid.on: pixel locations generated in the accumulation kernel
     unsigned long long *absmax = nullptr;  // max |z| (f64 bits) of the last accumulated image
+    unsigned *paths = nullptr;             // tiles per loop (TilePath) of the last accumulation
 };
 
 namespace {
@@ -78,6 +79,11 @@ int check_desc(const sarbp_plan_desc *d) {
     }
     if (d->tile_ppt < 0 || d->tile_ppt > 4 || d->tile_ppt == 1) {
         set_error("tile_ppt %d is not one of 0 (default), 2, 3, 4", d->tile_ppt);
+        return halide_error_code_constraint_violated;
+    }
+    if (d->row_windows < 0 || d->row_windows > 1) {
+        set_error("row_windows %d is not 0 (gathers) or 1 (windows where they fit)",
+                  d->row_windows);
         return halide_error_code_constraint_violated;
     }
     if (d->fft_blocks < 0) {
@@ -139,6 +145,7 @@ extern "C" int sarbp_plan_destroy(sarbp_plan *p) {
     (void)hipFree(p->pc);
     (void)hipFree(p->hdr);
     (void)hipFree(p->absmax);
+    (void)hipFree(p->paths);
     delete p;
     return 0;
 }
@@ -178,6 +185,7 @@ extern "C" int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out) {
               hipMalloc(&p->pc, sizeof(PulseConst) * (size_t)np) == hipSuccess &&
               hipMalloc(&p->hdr, sizeof(BpHeader)) == hipSuccess &&
               hipMalloc(&p->absmax, sizeof(unsigned long long)) == hipSuccess &&
+              hipMalloc(&p->paths, sizeof(unsigned) * kNumPaths) == hipSuccess &&
               (d->q_format != SARBP_Q_C16 ||
                hipMalloc(&p->qscale, sizeof(float) * (size_t)np) == hipSuccess);
     if (!ok) {
@@ -219,6 +227,22 @@ extern "C" int32_t sarbp_plan_grid_form(const sarbp_plan *p) {
     a.full_h = p->d.height;
     a.g = p->grid;
     return grid_cubic(a) ? SARBP_GRID_FORM_CUBIC : SARBP_GRID_FORM_PQ;
+}
+
+extern "C" int sarbp_plan_tile_paths(sarbp_plan *p, int64_t *counts) {
+    if (!p || !counts) {
+        set_error("sarbp_plan_tile_paths: NULL argument");
+        return halide_error_code_buffer_argument_is_null;
+    }
+    DeviceGuard g(p->d.device);
+    unsigned h[kNumPaths];
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(h, p->paths, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("sarbp_plan_tile_paths: copy failed");
+        return halide_error_code_copy_to_host_failed;
+    }
+    for (int k = 0; k < kNumPaths; ++k) counts[k] = h[k];
+    return 0;
 }
 
 extern "C" int sarbp_plan_img_output(sarbp_plan *p, const void *img_dev, double *dB_dev,
@@ -362,7 +386,7 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
     hipError_t e = launch_pulse_consts((const float *)pos_dev, (const float *)k_r_dev, d.nsamples,
                                        d.npulses, p->lay, d.norm_r0_policy,
                                        p->grid.on ? &p->grid : nullptr, p->pc, p->hdr, s,
-                                       p->absmax);
+                                       p->absmax, p->paths);
     if (e == hipSuccess) {
         BpArgs a{};
         a.q = q_dev ? q_dev : p->q;
@@ -391,6 +415,8 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
         a.out = (double2 *)out_dev;
         a.g = p->grid;
         a.absmax = p->absmax;
+        a.paths = p->paths;
+        a.row_windows = d.row_windows;
         e = launch_accumulate(a, s);
     }
     if (e != hipSuccess) {
@@ -438,7 +464,7 @@ int plan_prepare(sarbp_plan *p, const void *phs_dev, const void *k_r_dev, const 
     hipError_t e = launch_pulse_consts((const float *)pos_dev, (const float *)k_r_dev, d.nsamples,
                                        d.npulses, p->lay, d.norm_r0_policy,
                                        p->grid.on ? &p->grid : nullptr, p->pc, p->hdr, stream,
-                                       p->absmax);
+                                       p->absmax, p->paths);
     if (e != hipSuccess) {
         set_error("pulse-table launch failed: %s", hipGetErrorString(e));
         return halide_error_code_device_run_failed;
@@ -477,6 +503,8 @@ int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int6
     a.out = (double2 *)out_rows;
     a.g = p->grid;
     a.absmax = p->absmax;
+    a.paths = p->paths;
+    a.row_windows = d.row_windows;
     hipError_t e = launch_accumulate(a, stream);
     if (e != hipSuccess) {
         set_error("accumulation launch failed: %s", hipGetErrorString(e));

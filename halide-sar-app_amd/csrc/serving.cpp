@@ -27,7 +27,9 @@
 // A per-shape-class autotuner (shape class = N_fft, range-profile format, log2 of pulses and of
 // pixels) times the tile candidates on the first job of a class, over a band of its rows, each twice
 // in mirrored order, and uses the fastest for later jobs of that class; until the timings are in,
-// jobs use the library's size-based default tile.
+// jobs use the library's size-based default tile. A candidate is a tile height (16 x 32, 48 or 64)
+// and, for 16 x 48 and 16 x 64, the pulse-block form of the accumulation: row elements gathered
+// from HBM in blocks of 32 pulses, or row windows staged in LDS in blocks of 8 (the same bits).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -178,6 +180,7 @@ struct Job {
     int64_t pp = 0;  // pixel-pulses of the job
     int stream = 0;
     int ppt = 0;
+    int row_windows = 0;  // bands stage row windows in LDS (the tuner's choice)
     bool fused = false;
     bool small = false;
     bool prepared = false;       // image plane, pulse table and range profiles enqueued
@@ -208,8 +211,11 @@ struct FuseSlot {
     std::vector<Job *> group;  // jobs of the launch in flight
 };
 
+// A tuning candidate: tile_ppt plus kTuneWindows when the job's bands stage row windows in LDS.
+constexpr int kTuneWindows = 16;
+
 struct Tuning {
-    int best = 0;          // chosen tile_ppt (0: not decided)
+    int best = 0;          // chosen candidate (tile_ppt | kTuneWindows; 0: not decided)
     bool running = false;  // candidate timings in flight
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> probes;
 };
@@ -387,6 +393,7 @@ int prepare_job(sarbp_server *s, Job &j, hipStream_t st) {
     a.out = (double2 *)(b + j.ar.out);
     a.ppt = j.ppt;
     a.g = grid;
+    a.row_windows = j.row_windows;
     j.prepared = true;
     return 0;
 }
@@ -479,20 +486,23 @@ int run_alone(sarbp_server *s, Job &j) {
             j.d.nv, std::max<int64_t>(192, (4000000000LL / row_pp) / 192 * 192));
         // each candidate twice, in mirrored order, so that work on other streams that overlaps one
         // probe does not decide the class; the faster of its two times counts
-        for (int ppt : {4, 3, 2, 2, 3, 4}) {
+        constexpr int W = kTuneWindows;
+        for (int cand : {4, 3, 2, 4 | W, 3 | W, 3 | W, 4 | W, 2, 3, 4}) {
             hipEvent_t a, b;
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
             BpArgs args = j.args;
-            args.ppt = ppt;
+            args.ppt = cand & (W - 1);
+            args.row_windows = (cand & W) != 0;
             args.height = band;
             (void)hipEventRecord(a, st);
             e = launch_accumulate(args, st);
             (void)hipEventRecord(b, st);
-            tu.probes.push_back({ppt, {a, b}});
+            tu.probes.push_back({cand, {a, b}});
             if (e != hipSuccess) break;
         }
         j.ppt = default_tile_ppt(j.args.width, j.args.height);
+        j.row_windows = 0;
     }
     if (e != hipSuccess) {
         set_error("job %llu: accumulation failed: %s", (unsigned long long)j.id,
@@ -500,6 +510,7 @@ int run_alone(sarbp_server *s, Job &j) {
         return halide_error_code_device_run_failed;
     }
     j.args.ppt = j.ppt;
+    j.args.row_windows = j.row_windows;
     j.banding = true;
     j.next_row = 0;
     for (int k = 0; k < kBandsAhead; ++k)
@@ -633,7 +644,7 @@ void scheduler(sarbp_server *s) {
             bool done = true;
             for (auto &pr : tu.probes) done = done && event_done(pr.second.second);
             if (!done) continue;
-            std::map<int, float> fastest;  // tile_ppt -> its faster probe
+            std::map<int, float> fastest;  // candidate -> its faster probe
             for (auto &pr : tu.probes) {
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, pr.second.first, pr.second.second);
@@ -674,7 +685,8 @@ void scheduler(sarbp_server *s) {
             j.dev = s->arena_base + off;
             s->used_bytes += (int64_t)j.ar.total;
             Tuning &tu = s->tuning[j.cls];
-            j.ppt = tu.best ? tu.best : default_tile_ppt(j.d.nu, j.d.nv);
+            j.ppt = tu.best ? tu.best & (kTuneWindows - 1) : default_tile_ppt(j.d.nu, j.d.nv);
+            j.row_windows = (tu.best & kTuneWindows) != 0;
             // a small job of an untuned class runs alone once, on a big stream, to be timed
             const bool fuse = j.small && (tu.best != 0 || !s->cfg.autotune);
             j.small = fuse;
@@ -918,6 +930,7 @@ extern "C" int sarbp_server_wait(sarbp_server *s, uint64_t job_id, int32_t timeo
         stats->device_bytes = (int64_t)j->ar.total;
         stats->fused = j->fused ? 1 : 0;
         stats->tile_ppt = j->ppt;
+        stats->row_windows = j->row_windows;
         stats->status = j->status;
         stats->host_enqueue_us = (int32_t)std::min<int64_t>(j->host_ns / 1000, INT32_MAX);
         stats->device_us = (int32_t)std::min<double>(1000.0 * j->device_ms, (double)INT32_MAX);

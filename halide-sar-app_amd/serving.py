@@ -64,7 +64,8 @@ class JobDesc(ctypes.Structure):
 class JobStats(ctypes.Structure):
     _fields_ = [("t_submit_ns", _i64), ("t_admit_ns", _i64), ("t_end_ns", _i64),
                 ("device_bytes", _i64), ("fused", _i32), ("tile_ppt", _i32), ("status", _i32),
-                ("host_enqueue_us", _i32), ("device_us", _i32), ("reserved", _i32 * 3)]
+                ("host_enqueue_us", _i32), ("device_us", _i32), ("row_windows", _i32),
+                ("reserved", _i32 * 2)]
 
 
 _lib.sarbp_server_config_init.argtypes = [ctypes.POINTER(ServerConfig)]
@@ -287,7 +288,7 @@ def replay(server: Server, factory: JobFactory, rate_hz: float | None, keep_outp
             try:
                 st = server.wait(ids[i])
                 done[i] = (st.t_submit_ns, st.t_admit_ns, st.t_end_ns, st.fused, st.tile_ppt,
-                           st.host_enqueue_us, st.device_us)
+                           st.host_enqueue_us, st.device_us, st.row_windows)
             except Exception as e:  # noqa: BLE001 - reported after the stream
                 errors.append((i, str(e)))
             if i in keep:
@@ -314,9 +315,10 @@ def replay(server: Server, factory: JobFactory, rate_hz: float | None, keep_outp
         raise RuntimeError(f"{len(errors)} jobs failed, first: {errors[0]}")
     recs = []
     for i, sp in enumerate(factory.specs):
-        ts, ta, te, fused, ppt, hu, du = done[i]
+        ts, ta, te, fused, ppt, hu, du, rw = done[i]
         recs.append(dict(latency_ms=(te - ts) / 1e6, queue_ms=(ta - ts) / 1e6, t_end_ms=te / 1e6,
-                         t_submit_ms=ts / 1e6, fused=fused, ppt=ppt, host_enqueue_ms=hu / 1e3,
+                         t_submit_ms=ts / 1e6, fused=fused, ppt=ppt, row_windows=rw,
+                         host_enqueue_ms=hu / 1e3,
                          device_ms=du / 1e3,
                          pp=factory.pixel_pulses(i), big=sp.get("big", False)))
     return recs, wall, kept

@@ -145,7 +145,13 @@ typedef struct sarbp_plan_desc {
                                         > 0 = at most this many 256-thread blocks, each looping
                                         over profiles (N_fft <= 8192; for a device shared with
                                         other streams' accumulation launches). Same values. */
-    int32_t reserved[5];
+    int32_t row_windows;             /* grid mode, cubic form, complex-f32 rows, 16 x 48 and
+                                        16 x 64 tiles: 0 = every pixel gathers its row elements
+                                        from HBM, in blocks of 32 pulses (default); 1 = each tile
+                                        stages the row elements its pixels read in LDS, per block
+                                        of 8 pulses, where they fit (96 elements per pulse at
+                                        16 x 64, 80 at 16 x 48). The same image bit for bit. */
+    int32_t reserved[4];
 } sarbp_plan_desc;
 
 void sarbp_plan_desc_init(sarbp_plan_desc *d);
@@ -182,6 +188,12 @@ enum sarbp_grid_form {
     SARBP_GRID_FORM_CUBIC = 2  /* cubic in the row offset: 3 f64 FMAs per pixel-pulse */
 };
 int32_t sarbp_plan_grid_form(const sarbp_plan *p);
+
+/* How many tiles of the plan's last accumulation (since its pulse table was built) took each loop:
+ * counts[0] beyond the rows' cover (clamped, exact phase), [1] direct form, [2] expansion in its
+ * p, q form, [3] cubic form gathering row elements from HBM, [4] cubic form reading row windows
+ * staged in LDS. Synchronises the device. */
+int sarbp_plan_tile_paths(sarbp_plan *p, int64_t *counts);
 
 /* dB and u8 output (img_output_to_dB, img_output_u8) of the image the plan's last accumulation
  * wrote (sarbp_plan_execute / sarbp_plan_accumulate), in one pass over it: that accumulation also
@@ -277,7 +289,8 @@ typedef struct sarbp_job_stats {
     int32_t host_enqueue_us;                   /* scheduler time spent enqueuing the job */
     int32_t device_us;                         /* device time from the job's first operation to
                                                   its output in host memory */
-    int32_t reserved[3];
+    int32_t row_windows;                       /* bands staged row windows in LDS */
+    int32_t reserved[2];
 } sarbp_job_stats;
 
 void sarbp_server_config_init(sarbp_server_config *cfg);

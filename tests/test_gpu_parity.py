@@ -333,6 +333,56 @@ def test_grid_cubic_sheared_axes(sar, orc, scenes, pipe, torch_dev):
     assert e < TOL
 
 
+@pytest.mark.parametrize("case", ["c2", "c2_ppt3", "c2_spacing1.33", "swapped", "sheared"])
+def test_row_windows_same_bits(sar, scenes, pipe, torch_dev, case):
+    """Grid mode, cubic form: tiles reading their row elements from windows staged in LDS give the
+    image of the gathering kernel bit for bit. On the C2 grid every tile's window fits (16 x 64
+    tiles: 71-73 elements per pulse against a 96-element slot; 16 x 48: 68-70 against 80); at 1.33
+    times the pixel spacing the row coordinates span 94-98 elements per tile and on the sheared
+    grid (v_hat turned 6 degrees towards u_hat) 94-97, so part of the tiles keep the gathers; with
+    u_hat and v_hat swapped the range runs along the tiles' 64-pixel side and no window fits. The
+    tile counters say which loop ran."""
+    torch = torch_dev
+    sc = scenes.config_scene("C2")
+    base = pipe.scene_grid(sc, with_pixel_locs=False)
+    uh, vh = np.asarray(base["u_hat"], float), np.asarray(base["v_hat"], float)
+    f, ppt = 1.0, 4
+    if case.startswith("c2_spacing"):
+        f = float(case[len("c2_spacing"):])
+    if case == "c2_ppt3":
+        ppt = 3
+    if case == "sheared":
+        ang = np.deg2rad(6.0)
+        vh = np.cos(ang) * vh + np.sin(ang) * uh
+    if case == "swapped":
+        uh, vh = vh, uh
+    nu, nv = base["nu"], base["nv"]
+    grid = sar.image_grid(nu, nv, base["d_u"] * f, base["d_v"] * f, uh, vh, with_pixel_locs=False)
+    ds = pipe.DeviceScene(sc, grid)
+    outs, paths = [], []
+    for rw in (1, 0):  # windows where they fit, then gathers
+        plan = ds.plan(tile_ppt=ppt, row_windows=rw)
+        assert plan.grid_form == sar.GRID_FORM_CUBIC
+        out = ds.output()
+        ds.run(plan, out)
+        paths.append(plan.tile_paths())
+        outs.append(out)
+    torch.cuda.synchronize()
+    print(f"row windows, {case}: tile paths with windows {paths[0]}, without {paths[1]}")
+    assert torch.equal(outs[0], outs[1])
+    assert bool(torch.isfinite(outs[0].abs()).all())
+    ntiles = ((nu + 15) // 16) * ((nv + 16 * ppt - 1) // (16 * ppt))
+    assert sum(paths[0].values()) == ntiles == sum(paths[1].values())
+    assert paths[1]["cubic_window"] == 0
+    assert paths[0]["cubic_window"] + paths[0]["cubic_gather"] == paths[1]["cubic_gather"]
+    if case in ("c2", "c2_ppt3"):
+        assert paths[0]["cubic_window"] == ntiles
+    if case in ("c2_spacing1.33", "sheared"):
+        assert paths[0]["cubic_window"] > 0 and paths[0]["cubic_gather"] > 0
+    if case == "swapped":
+        assert paths[0]["cubic_window"] == 0 and paths[0]["cubic_gather"] > 0
+
+
 def test_c3_full_size_sampled(sar, orc, scenes, pipe, torch_dev):
     """Configuration C3 at full size (8192^2 x 8000 pulses x 4096 samples, N_fft 16384: the largest
     FFT held in LDS), pixel locations generated on the device: oracle on 1536 sampled pixels

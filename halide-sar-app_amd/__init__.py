@@ -178,6 +178,7 @@ _lib.sarbp_plan_tile_paths.argtypes = [_P, _P]
 _lib.sarbp_plan_read_pulse_norms.argtypes = [_P, _P, _i32]
 _lib.sarbp_plan_read_profiles.argtypes = [_P, _P]
 _lib.sarbp_img_output_device.argtypes = [_P, _i64, _P, _P, _f64, _f64, _P]
+_lib.sarbp_img_output_device_noalloc.argtypes = [_P, _i64, _P, _P, _f64, _f64, _P]
 _lib.sarbp_sim_phs_device.argtypes = [_P, _i32, _P, _i32, _P, _P, _i32, _P, _P]
 _lib.sarbp_pixel_locs_device.argtypes = [_P, _i32, _P, _i32, _P, _P, _P, _P]
 
@@ -471,6 +472,15 @@ def img_output_device(img, npix: int, dB=None, u8=None, dB_min=0.0, dB_max=1.0, 
                                       float(dB_max), _stream_handle(stream))
     if rv != 0:
         raise RuntimeError(f"sarbp_img_output_device failed ({rv}): {last_error()}")
+
+
+def img_output_device_noalloc(img, npix: int, dB=None, u8=None, dB_min=0.0, dB_max=1.0,
+                              stream=None):
+    """sarbp_img_output_device_noalloc: the same output without allocating device memory."""
+    rv = _lib.sarbp_img_output_device_noalloc(_ptr(img), int(npix), _ptr(dB), _ptr(u8),
+                                              float(dB_min), float(dB_max), _stream_handle(stream))
+    if rv != 0:
+        raise RuntimeError(f"sarbp_img_output_device_noalloc failed ({rv}): {last_error()}")
 
 
 def sim_phs_device(pos64, k_r64, targets, amps, phs, stream=None):

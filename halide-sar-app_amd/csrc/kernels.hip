@@ -23,6 +23,7 @@
 #include <utility>
 #include <vector>
 
+#include "img_math.h"
 #include "internal.h"
 #include "tile_order.h"
 
@@ -136,15 +137,6 @@ __device__ inline double row_coord(double s, double B) {
 __device__ inline double grid_coord(double uh, double u, double vh, double v) {
 #pragma clang fp contract(off)
     return (0.0 + uh * u) + vh * v;
-}
-
-// |z|^2 rounded as the reference rounds it (abs of a ComplexFunc: re * re + im * im, each product
-// and the sum rounded once; img_output.cpp:24-26). Every max and every dB value of this library
-// takes |z| as the correctly rounded sqrt of this, so a maximum found anywhere (the accumulation's
-// epilogue, absmax_kernel) equals the |z| the dB kernel forms for the same pixel.
-__device__ inline double abs2(double2 z) {
-#pragma clang fp contract(off)
-    return z.x * z.x + z.y * z.y;
 }
 
 // Raises *cell to the bit pattern of sqrt(m2) (m2 >= 0 orders like its bits), skipping the atomic
@@ -1568,21 +1560,6 @@ __global__ __launch_bounds__(kImgThreads) void absmax_kernel(const double2 *__re
         // most blocks' maxima are below the running one by now: their atomics are skipped
         raise_absmax(out, m);
     }
-}
-
-__device__ inline uint8_t to_u8(double dB, double lo, double hi) {
-    // normalize (util.h:13-15), dB_scale (signal.h:55-57), u8 truncation (img_output.cpp:50)
-    double c = dB < hi ? dB : hi;
-    c = c > lo ? c : lo;
-    double s = (c - lo) / (hi - lo) * 255.0;
-    s = s < 255.0 ? s : 255.0;
-    s = s > 0.0 ? s : 0.0;
-    return (s == s) ? (uint8_t)s : (uint8_t)0;
-}
-
-// 10 * log10(|z| / m), log10(x) = log(x) / log(Expr(10)) with the f32 log(10) (util.h:56-58)
-__device__ inline double to_db(double2 z, double m, double ln10f) {
-    return 10.0 * (log(sqrt(abs2(z)) / m) / ln10f);
 }
 
 // Each thread converts 4 consecutive pixels: four 16-byte loads, two 16-byte dB stores and one

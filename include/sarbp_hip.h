@@ -228,6 +228,16 @@ int sarbp_plan_read_pulse_norms(sarbp_plan *p, double *host_out, int32_t n);
 int sarbp_img_output_device(const void *img_dev, int64_t npix, double *dB_dev, uint8_t *u8_dev,
                             double dB_min, double dB_max, void *stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops): sarbp_img_output_device's dB and u8 output without any
+ * allocation. The max |z| is reduced through a static device array of the library (8200 bytes per
+ * GPU, held until the process ends; it does not grow with npix, so no image is too large for it).
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU (on different streams
+ * or host threads) race; calls ordered on one stream are safe. Asynchronous on `stream`; the same
+ * values as sarbp_img_output_device. */
+int sarbp_img_output_device_noalloc(const void *img_dev, int64_t npix, double *dB_dev,
+                                    uint8_t *u8_dev, double dB_min, double dB_max, void *stream);
+
 /* Synthetic phase history on the device: phs[p][k] = sum_s amp_s exp(j k_r64[k] * D[p][s]) with
  * D[p][s] = |pos_p| - |pos_p - tgt_s| in f64. pos f64 [np][3], tgt f64 [nt][3], amp complex f64
  * [nt], k_r64 f64 [ns]; out complex f32 [np][ns]. */

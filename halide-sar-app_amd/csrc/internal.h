@@ -76,13 +76,18 @@ hipError_t launch_pulse_consts(const float *pos, const float *k_r, int ns, int n
                                const RowLayout &lay, int norm_policy, const PixelGrid *grid,
                                PulseConst *pc, BpHeader *hdr, hipStream_t stream,
                                unsigned long long *absmax = nullptr, unsigned *paths = nullptr);
+// N_fft above kMaxFft (up to kMaxFftTwoPass) takes a four-step FFT through `scratch`
+// (two_pass_scratch_elems(N_fft, np) complex f32 elements; ignored for smaller N_fft).
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
                                  const double *wp, int ns, int np, const RowLayout &lay,
                                  const float2 *tw, float2 *rot, void *q, int q_format,
-                                 float *qscale, hipStream_t stream, int shared_blocks = 0);
+                                 float *qscale, hipStream_t stream, int shared_blocks = 0,
+                                 float2 *scratch = nullptr, size_t scratch_elems = 0);
 // shared_blocks > 0 (and N_fft <= kMaxFftShared): the profiles are computed by at most that many
 // 256-thread blocks, which suits a device busy with accumulation launches of other streams.
 constexpr int kMaxFftShared = 8192;
+// Scratch the four-step FFT of N_fft > kMaxFft needs (complex f32 elements; 0 up to kMaxFft).
+size_t two_pass_scratch_elems(int n_fft, int np);
 struct BpArgs {
     const void *q;
     int q_format;
@@ -160,7 +165,11 @@ hipError_t launch_sim_phs(const double *pos, int np, const double *k64, int ns,
 const float2 *twiddles(int device, int n_fft, hipStream_t stream);
 void release_twiddles();
 
+// Longest FFT done in one LDS row (one workgroup per profile), and the longest done in two passes
+// through HBM: rows of 1.5 N_fft + 2 elements must index in 18 bits (the fixed-point row
+// coordinate of the accumulation).
 constexpr int kMaxFft = 16384;
+constexpr int kMaxFftTwoPass = 131072;
 
 // Plan steps for callers that split the accumulation into row bands (the Halide-buffer entry point
 // overlaps transfers with computation this way). plan_prepare enqueues the range profiles and the

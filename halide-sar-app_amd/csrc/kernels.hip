@@ -288,6 +288,26 @@ __device__ inline int profile_source(int p, int np) {
     return (int)(((long long)p + 2LL * pmid) % np);
 }
 
+// Sample x of the FFT input of a pulse whose phase-history row is `row` and whose pulse window is
+// wps: the padded, windowed, fftshifted sample.
+__device__ inline float2 profile_sample(const float2 *__restrict__ row,
+                                        const float *__restrict__ k_r,
+                                        const double *__restrict__ ws, double wps, int ns, int N,
+                                        int xpad, int x) {
+    const int xs = (x + N / 2) & (N - 1);  // fftshift (signal_complex.h:24-29)
+    const int i = xs - xpad;
+    float2 v = make_float2(0.f, 0.f);
+    if (i >= 0 && i < ns) {
+        const float2 a = row[i];
+        const float f = fabsf(k_r[i]);      // filt = |k_r| (backprojection.cpp:85)
+        const double w = ws[i] * wps;       // win (backprojection.cpp:82)
+        // phs * filt rounds in f32, then the f64 window applies (backprojection.cpp:90)
+        v.x = (float)((double)(a.x * f) * w);
+        v.y = (float)((double)(a.y * f) * w);
+    }
+    return v;
+}
+
 // Fills buf[0, N) with the padded, windowed, fftshifted samples of pulse s.
 template <int T>
 __device__ inline void load_profile_row(float2 *buf, const float2 *__restrict__ phs,
@@ -295,24 +315,10 @@ __device__ inline void load_profile_row(float2 *buf, const float2 *__restrict__ 
                                         const double *__restrict__ ws,
                                         const double *__restrict__ wp, int ns, int N, int s,
                                         int tid) {
-    const int half = N / 2;                 // ceil(N/2) for even N
     const int xpad = floordiv2(N - ns);     // pad's leading offset (signal_complex.h:14)
     const double wps = wp[s];
     const float2 *row = phs + (size_t)s * ns;
-    for (int x = tid; x < N; x += T) {
-        const int xs = (x + half) & (N - 1);  // fftshift (signal_complex.h:24-29)
-        const int i = xs - xpad;
-        float2 v = make_float2(0.f, 0.f);
-        if (i >= 0 && i < ns) {
-            const float2 a = row[i];
-            const float f = fabsf(k_r[i]);      // filt = |k_r| (backprojection.cpp:85)
-            const double w = ws[i] * wps;       // win (backprojection.cpp:82)
-            // phs * filt rounds in f32, then the f64 window applies (backprojection.cpp:90)
-            v.x = (float)((double)(a.x * f) * w);
-            v.y = (float)((double)(a.y * f) * w);
-        }
-        buf[x] = v;
-    }
+    for (int x = tid; x < N; x += T) buf[x] = profile_sample(row, k_r, ws, wps, ns, N, xpad, x);
 }
 
 // Writes the transformed row buf[0, N) as profile row p: Q(k) = dft((k + N/2) mod N)
@@ -435,6 +441,133 @@ __global__ __launch_bounds__(kFftSharedThreads) void range_profile_shared_kernel
         }
         store_profile_row<QT, kFftSharedThreads>(a, red, N, rot, ext, len, q, qscale, p, tid);
     }
+}
+
+// Range profiles longer than one LDS row (kMaxFft < N <= kMaxFftTwoPass): a four-step FFT of
+// N = N1 N2 (N1 = 2^ceil(m/2), N2 = 2^floor(m/2)) in three launches over a batch of profiles, with
+// n = n2 + N2 n1 and k = k1 + N1 k2:
+//   fft_cols_kernel  per profile and block of C = kMaxFft / N1 consecutive n2: the length-N1 DFTs
+//                    over n1 of x[n2 + N2 n1] in LDS, times W_N^(n2 k1), into Y[n2][k1]
+//   fft_rows_kernel  per profile and block of D = kMaxFft / N2 consecutive k1: the length-N2 DFTs
+//                    over n2 of Y[n2][k1] in LDS, into X[k1 + N1 k2] (natural order)
+//   rows_store_kernel per profile: the rotated (A, B) row elements from X, as store_profile_row
+// Sequences sit in LDS at a stride of L + 1 elements, so the transposed accesses of the loads and
+// stores hit distinct banks while global accesses stay contiguous. The butterflies are those of
+// fft_pass; the twiddles come from the N-point table.
+constexpr int kTwoPassThreads = kFftThreads;
+
+// One Stockham pass of radix R over kMaxFft / L sequences of length L = 2^logL at stride L + 1:
+// the butterflies of fft_pass, applied to each sequence (twiddle W_L^t = tw[t * N / L]).
+template <int R>
+__device__ inline void fft_pass_multi(float2 *buf, int logL, int Ns, int tw_scale,
+                                      const float2 *__restrict__ tw, int tid) {
+    constexpr int NB = kMaxFft / R / kTwoPassThreads;
+    const int L = 1 << logL, nbf = L / R, lp = L + 1;
+    int lognbf = 0;
+    while ((1 << lognbf) < nbf) ++lognbf;
+    const int stride = L / (Ns * R);
+    float2 v[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = tid + b * kTwoPassThreads;
+        const int c = j >> lognbf, jj = j & (nbf - 1);
+        const float2 *src = buf + c * lp + jj;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = src[r * nbf];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = tid + b * kTwoPassThreads;
+        const int c = j >> lognbf, jj = j & (nbf - 1);
+        const int k = jj & (Ns - 1);
+        if (k != 0) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * k * stride * tw_scale]);
+        }
+        dft_r<R>(v[b]);
+        float2 *dst = buf + c * lp + (jj - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[r * Ns] = v[b][r];
+    }
+    __syncthreads();
+}
+
+// The length-2^logL DFTs of every sequence in LDS, in place (radix 4, one radix-2 pass first for
+// an odd logL).
+__device__ inline void fft_multi(float2 *buf, int logL, int logN, const float2 *__restrict__ tw,
+                                 int tid) {
+    const int tw_scale = 1 << (logN - logL);
+    int Ns = 1;
+    if (logL & 1) {
+        fft_pass_multi<2>(buf, logL, 1, tw_scale, tw, tid);
+        Ns = 2;
+    }
+    for (; Ns < (1 << logL); Ns *= 4) fft_pass_multi<4>(buf, logL, Ns, tw_scale, tw, tid);
+}
+
+__global__ __launch_bounds__(kTwoPassThreads) void fft_cols_kernel(
+    const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
+    const double *__restrict__ wp, int ns, int np, int logN, int logN1, int p0,
+    const float2 *__restrict__ tw, float2 *__restrict__ Y) {
+    extern __shared__ float2 buf[];
+    const int N = 1 << logN, N1 = 1 << logN1, N2 = N >> logN1;
+    const int logC = 14 - logN1, C = 1 << logC;  // kMaxFft = 2^14 elements per block
+    const int blocks_per_profile = N2 >> logC;
+    const int pl = blockIdx.x / blocks_per_profile, c0 = (blockIdx.x % blocks_per_profile) << logC;
+    const int p = p0 + pl, s = profile_source(p, np);
+    const int xpad = floordiv2(N - ns);
+    const double wps = wp[s];
+    const float2 *row = phs + (size_t)s * ns;
+    const int tid = threadIdx.x;
+    // x[n2 + N2 n1], consecutive threads along n2 (contiguous input)
+    for (int e = tid; e < kMaxFft; e += kTwoPassThreads) {
+        const int c = e & (C - 1), n1 = e >> logC;
+        buf[c * (N1 + 1) + n1] = profile_sample(row, k_r, ws, wps, ns, N, xpad, c0 + c + N2 * n1);
+    }
+    __syncthreads();
+    fft_multi(buf, logN1, logN, tw, tid);
+    // Y[n2][k1] = DFT * W_N^(n2 k1), consecutive threads along k1
+    float2 *out = Y + (size_t)pl * N;
+    for (int e = tid; e < kMaxFft; e += kTwoPassThreads) {
+        const int k1 = e & (N1 - 1), c = e >> logN1, n2 = c0 + c;
+        out[(size_t)n2 * N1 + k1] = cmul(buf[c * (N1 + 1) + k1], tw[(n2 * k1) & (N - 1)]);
+    }
+}
+
+__global__ __launch_bounds__(kTwoPassThreads) void fft_rows_kernel(
+    const float2 *__restrict__ Y, int logN, int logN1, const float2 *__restrict__ tw,
+    float2 *__restrict__ X) {
+    extern __shared__ float2 buf[];
+    const int N = 1 << logN, N1 = 1 << logN1, logN2 = logN - logN1, N2 = 1 << logN2;
+    const int logD = 14 - logN2, D = 1 << logD;
+    const int blocks_per_profile = N1 >> logD;
+    const int pl = blockIdx.x / blocks_per_profile, d0 = (blockIdx.x % blocks_per_profile) << logD;
+    const float2 *in = Y + (size_t)pl * N;
+    const int tid = threadIdx.x;
+    // Y[n2][d0 + d], consecutive threads along d
+    for (int e = tid; e < kMaxFft; e += kTwoPassThreads) {
+        const int d = e & (D - 1), n2 = e >> logD;
+        buf[d * (N2 + 1) + n2] = in[(size_t)n2 * N1 + d0 + d];
+    }
+    __syncthreads();
+    fft_multi(buf, logN2, logN, tw, tid);
+    // X[d0 + d + N1 k2], consecutive threads along d
+    float2 *out = X + (size_t)pl * N;
+    for (int e = tid; e < kMaxFft; e += kTwoPassThreads) {
+        const int d = e & (D - 1), k2 = e >> logD;
+        out[(size_t)k2 * N1 + d0 + d] = buf[d * (N2 + 1) + k2];
+    }
+}
+
+template <typename QT>
+__global__ __launch_bounds__(kTwoPassThreads) void rows_store_kernel(
+    const float2 *__restrict__ X, int N, int p0, const float2 *__restrict__ rot, int ext, int len,
+    QT *__restrict__ q, float *__restrict__ qscale) {
+    __shared__ float red[kTwoPassThreads / 64];
+    const int pl = blockIdx.x;
+    store_profile_row<QT, kTwoPassThreads>(X + (size_t)pl * N, red, N, rot, ext, len, q, qscale,
+                                           p0 + pl, threadIdx.x);
 }
 
 // ------------------------------------------------------------------ accumulation
@@ -606,7 +739,7 @@ __device__ inline void accumulate_pulses(const BpArgs &a, const PixTerms<PPT> &P
 constexpr int kExpBlock = 128;
 constexpr double kExpandTol = 1e-8;
 constexpr double kRowMagic = 1572864.0;  // 1.5 * 2^20
-static_assert(kMaxFft + kMaxFft / 2 + 2 < (1 << 18), "rows fit the fixed-point index");
+static_assert(kMaxFftTwoPass + kMaxFftTwoPass / 2 + 2 < (1 << 18), "rows fit the fixed-point index");
 // Row coordinates in fixed point: the expansion's T carries 1.5 * 2^20, so that the f64 result
 // y = t + 1.5 * 2^20 (t in [-2^19, 2^19)) has exponent 20 and its mantissa is (t + 2^19) * 2^32: the
 // high word's low 19 bits hold floor(t) for t >= 0 (a t in [-2^18, 0) gives an index of at least
@@ -1713,15 +1846,66 @@ static void allow_max_dynamic_lds(const void *fn) {
     (void)hipGetLastError();
 }
 
+// Four-step range profiles (kMaxFft < N): batches of profiles through the column, row and store
+// kernels, with Y and X of a batch in `scratch` (2 batch N elements).
+template <typename QT>
+static hipError_t launch_two_pass(const float2 *phs, const float *k_r, const double *ws,
+                                  const double *wp, int ns, int np, int logN, const RowLayout &lay,
+                                  const float2 *tw, const float2 *rot, QT *q, float *qscale,
+                                  float2 *scratch, size_t scratch_elems, hipStream_t stream) {
+    const int N = 1 << logN, logN1 = (logN + 1) / 2, N2 = N >> logN1;
+    const int batch = (int)std::min<size_t>((size_t)np, scratch_elems / (2 * (size_t)N));
+    if (batch < 1) return hipErrorInvalidValue;
+    const size_t lds = sizeof(float2) * (size_t)(kMaxFft + kMaxFft / N2);  // sequences at L + 1
+    static std::once_flag once;
+    std::call_once(once, [] {
+        allow_max_dynamic_lds((const void *)&fft_cols_kernel);
+        allow_max_dynamic_lds((const void *)&fft_rows_kernel);
+    });
+    float2 *Y = scratch, *X = scratch + (size_t)batch * N;
+    const int blocks = N / kMaxFft;  // per profile, in both passes
+    for (int p0 = 0; p0 < np; p0 += batch) {
+        const int nb = std::min(batch, np - p0);
+        hipLaunchKernelGGL(fft_cols_kernel, dim3(nb * blocks), dim3(kTwoPassThreads), lds, stream,
+                           phs, k_r, ws, wp, ns, np, logN, logN1, p0, tw, Y);
+        hipLaunchKernelGGL(fft_rows_kernel, dim3(nb * blocks), dim3(kTwoPassThreads), lds, stream,
+                           (const float2 *)Y, logN, logN1, tw, X);
+        hipLaunchKernelGGL(rows_store_kernel<QT>, dim3(nb), dim3(kTwoPassThreads), 0, stream,
+                           (const float2 *)X, N, p0, rot, lay.ext, lay.len, q, qscale);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+size_t two_pass_scratch_elems(int n_fft, int np) {
+    if (n_fft <= kMaxFft) return 0;
+    // batches of up to 2^27 elements (1 GiB) of Y and X together, at least one profile
+    const size_t per = 2 * (size_t)n_fft;
+    const size_t batch = std::max<size_t>(1, std::min<size_t>((size_t)np, ((size_t)1 << 27) / per));
+    return batch * per;
+}
+
 hipError_t launch_range_profiles(const void *phs, const float *k_r, const double *ws,
                                  const double *wp, int ns, int np, const RowLayout &lay,
                                  const float2 *tw, float2 *rot, void *q, int q_format,
-                                 float *qscale, hipStream_t stream, int shared_blocks) {
+                                 float *qscale, hipStream_t stream, int shared_blocks,
+                                 float2 *scratch, size_t scratch_elems) {
     const int n_fft = lay.n_fft;
     int logN = 0;
     while ((1 << logN) < n_fft) ++logN;
     hipLaunchKernelGGL(rotation_kernel, dim3((lay.len + 255) / 256), dim3(256), 0, stream, k_r, ns,
                        lay.lsa, lay.lsr, lay.ext, lay.len, rot);
+    if (n_fft > kMaxFft) {
+        if (n_fft > kMaxFftTwoPass || !scratch) return hipErrorInvalidValue;
+        if (q_format == SARBP_Q_C16)
+            return launch_two_pass<QPairF16>((const float2 *)phs, k_r, ws, wp, ns, np, logN, lay,
+                                             tw, rot, (QPairF16 *)q, qscale, scratch,
+                                             scratch_elems, stream);
+        return launch_two_pass<QPairF32>((const float2 *)phs, k_r, ws, wp, ns, np, logN, lay, tw,
+                                         rot, (QPairF32 *)q, nullptr, scratch, scratch_elems,
+                                         stream);
+    }
     if (shared_blocks > 0 && n_fft <= kMaxFftShared) {
         const size_t lds = 2 * sizeof(float2) * (size_t)n_fft;
         const dim3 grid((unsigned)std::min(np, shared_blocks));

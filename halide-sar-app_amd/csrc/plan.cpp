@@ -26,6 +26,8 @@ struct sarbp_plan {
     PixelGrid grid{};           // grid.on: pixel locations generated in the accumulation kernel
     unsigned long long *absmax = nullptr;  // max |z| (f64 bits) of the last accumulated image
     unsigned *paths = nullptr;             // tiles per loop (TilePath) of the last accumulation
+    float2 *fft_scratch = nullptr;         // four-step FFT work space (N_fft > kMaxFft)
+    size_t fft_scratch_elems = 0;
 };
 
 namespace {
@@ -59,10 +61,10 @@ int check_desc(const sarbp_plan_desc *d) {
                   d->nsamples, d->npulses);
         return halide_error_code_param_too_small;
     }
-    if (!is_pow2(d->n_fft) || d->n_fft < 2 || d->n_fft > kMaxFft) {
-        set_error("N_fft (%d) must be a power of two in [2, %d]", d->n_fft, kMaxFft);
-        return d->n_fft > kMaxFft ? halide_error_code_param_too_large
-                                  : halide_error_code_param_too_small;
+    if (!is_pow2(d->n_fft) || d->n_fft < 2 || d->n_fft > kMaxFftTwoPass) {
+        set_error("N_fft (%d) must be a power of two in [2, %d]", d->n_fft, kMaxFftTwoPass);
+        return d->n_fft > kMaxFftTwoPass ? halide_error_code_param_too_large
+                                         : halide_error_code_param_too_small;
     }
     if (d->nu < 1 || d->nv < 1 || d->width < 0 || d->height < 0 || d->r_extent < 0) {
         set_error("image extents must be positive and region extents non-negative");
@@ -146,6 +148,7 @@ extern "C" int sarbp_plan_destroy(sarbp_plan *p) {
     (void)hipFree(p->hdr);
     (void)hipFree(p->absmax);
     (void)hipFree(p->paths);
+    (void)hipFree(p->fft_scratch);
     delete p;
     return 0;
 }
@@ -186,6 +189,8 @@ extern "C" int sarbp_plan_create(const sarbp_plan_desc *d, sarbp_plan **out) {
               hipMalloc(&p->hdr, sizeof(BpHeader)) == hipSuccess &&
               hipMalloc(&p->absmax, sizeof(unsigned long long)) == hipSuccess &&
               hipMalloc(&p->paths, sizeof(unsigned) * kNumPaths) == hipSuccess &&
+              ((p->fft_scratch_elems = two_pass_scratch_elems(N, np)) == 0 ||
+               hipMalloc(&p->fft_scratch, sizeof(float2) * p->fft_scratch_elems) == hipSuccess) &&
               (d->q_format != SARBP_Q_C16 ||
                hipMalloc(&p->qscale, sizeof(float) * (size_t)np) == hipSuccess);
     if (!ok) {
@@ -364,7 +369,8 @@ extern "C" int sarbp_plan_range_profiles(sarbp_plan *p, const void *phs_dev, con
     hipError_t e = launch_range_profiles(phs_dev, (const float *)k_r_dev, p->ws, p->wp,
                                          p->d.nsamples, p->d.npulses, p->lay, p->tw, p->rot,
                                          q_dev ? q_dev : p->q, p->d.q_format, p->qscale,
-                                         (hipStream_t)stream, p->d.fft_blocks);
+                                         (hipStream_t)stream, p->d.fft_blocks, p->fft_scratch,
+                                         p->fft_scratch_elems);
     if (e != hipSuccess) {
         set_error("range-profile launch failed: %s", hipGetErrorString(e));
         return halide_error_code_device_run_failed;

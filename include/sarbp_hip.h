@@ -38,7 +38,8 @@ extern "C" {
  *   phs        f32 {2, nsamples, npulses}          (PlatformData.cpp:238)
  *   k_r        f32 {nsamples}                      (PlatformData.cpp:137)
  *   taylor_s_l Taylor sidelobe level in dB         (backprojection.cpp:37)
- *   N_fft      FFT length (power of two, 2..16384) (backprojection.cpp:38, sarbp.cpp:258)
+ *   N_fft      FFT length (power of two, 2..131072; above 16384 a four-step FFT through
+ *              HBM) (backprojection.cpp:38, sarbp.cpp:258)
  *   delta_r    range resolution                    (backprojection.cpp:39)
  *   u, v       f64 {nu}, {nv}; only extents used   (backprojection.cpp:71-72)
  *   pos        f32 {3, npulses}                    (PlatformData.cpp:219)

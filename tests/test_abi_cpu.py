@@ -76,7 +76,7 @@ def test_validation_error_codes(sar):
     assert rv == E["bad_type"]
     # N_fft not a power of two / too large / too small
     assert sar.backprojection(phs, k_r, 17, 100, 0.2, u, v, pos, r, out) == E["param_too_small"]
-    assert sar.backprojection(phs, k_r, 17, 1 << 15, 0.2, u, v, pos, r, out) == E["param_too_large"]
+    assert sar.backprojection(phs, k_r, 17, 1 << 18, 0.2, u, v, pos, r, out) == E["param_too_large"]
     # k_r shorter than nsamples: access out of bounds
     assert sar.backprojection(phs, k_r[:10], 17, 64, 0.2, u, v, pos, r, out) == \
         E["access_out_of_bounds"]

@@ -165,6 +165,50 @@ def test_range_profiles_vs_oracle(sar, orc, scenes, torch_dev):
         assert err < 1e-5
 
 
+@pytest.mark.parametrize("ns,np_,q_format", [(8192, 12, 0), (8192, 7, 1), (20000, 3, 0),
+                                             (40000, 3, 0)])
+def test_range_profiles_four_step(sar, orc, scenes, torch_dev, ns, np_, q_format):
+    """Range profiles longer than one LDS row take the four-step FFT through HBM: N_fft 32768 (the
+    reference's own 8192-sample GPU run, sarbp.cpp:258 with upsample 2), 65536 and 131072, the
+    largest whose rows the accumulation can index; odd pulse counts (the pulse-axis quirk) and
+    both row formats. Q against the oracle's FP64 radix-2 FFT."""
+    torch = torch_dev
+    n_fft = sar.nfft(ns, 2)
+    assert n_fft > 16384
+    sc = scenes.tiny_scene(npulses=np_, nsamples=ns, seed=ns + np_)
+    plan = sar.Plan(sc.nsamples, sc.npulses, n_fft, sc.taylor, sc.delta_r, 8, 8, q_format=q_format)
+    print(f"four-step FFT: ns={ns} np={np_} N={n_fft} format={q_format}", flush=True)
+    plan.range_profiles(torch.from_numpy(sc.phs).cuda(), torch.from_numpy(sc.k_r).cuda())
+    got = plan.read_profiles()
+    ref = orc.range_profiles(sc.phs, sc.k_r, sc.taylor, n_fft)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"   Q vs oracle: max err / max |Q| = {err:.3e}")
+    assert err < (1e-5 if q_format == 0 else 2e-3)
+
+
+def test_backprojection_n_fft_32768_vs_oracle(sar, orc, scenes, torch_dev):
+    """backprojection() at N_fft 32768 (8192 samples, upsample 2) through the Halide-buffer entry
+    point, with 2050 pulses: more than one batch of the four-step FFT's scratch (2048 profiles of
+    32768 points each), on a 48 x 48 grid around the scene; the whole image against the oracle."""
+    sc = scenes.circular_arc_scene(npulses=2050, nsamples=8192, f_c=9.6e9, df=0.1e6, radius=2e3,
+                                   altitude=1.5e3, arc_deg=6.0, ntargets=3, patch=8.0,
+                                   snr_db=20.0, taylor=17, upsample=2, seed=8192, name="n32768")
+    sc.phs = scenes.simulate_phs_numpy(sc)
+    n_fft = sar.nfft(sc.nsamples, 2)
+    assert n_fft == 32768
+    ip = sar.img_plane_create(sc.nsamples, sc.npulses, sc.delta_r, sc.R_c, with_pixel_locs=False)
+    nu = nv = 48
+    grid = sar.image_grid(nu, nv, sc.delta_r, sc.delta_r, ip["u_hat"], ip["v_hat"])
+    print(f"backprojection N={n_fft} np={sc.npulses} {nu}x{nv}", flush=True)
+    img = run_abi(sar, sc.phs, sc.k_r, sc.taylor, n_fft, sc.delta_r, nu, nv, sc.pos,
+                  grid["pixel_locs"])
+    ref = orc.backprojection(sc.phs, sc.k_r, sc.taylor, n_fft, sc.delta_r, nu, nv, sc.pos,
+                             grid["pixel_locs"], nthreads=NTHREADS)
+    e = rel_l2(img, ref)
+    print(f"   rel_l2 {e:.3e}")
+    assert np.all(np.isfinite(img)) and e < TOL
+
+
 @pytest.mark.parametrize("q_format", [0, 1])
 def test_range_profiles_shared_kernel_same_bits(sar, orc, scenes, torch_dev, q_format):
     """The 256-thread range-profile variant (fft_blocks > 0, used by the serving engine) writes the

@@ -449,7 +449,7 @@ class Plan:
         """GRID_FORM_NONE / _PQ / _CUBIC: the expansion form of the plan's grid-mode tiles."""
         return int(_lib.sarbp_plan_grid_form(self._p))
 
-    TILE_PATHS = ("exact", "direct", "pq", "cubic_gather", "cubic_window")
+    TILE_PATHS = ("exact", "direct", "pq", "cubic_gather", "cubic_window", "cubic_df")
 
     def tile_paths(self) -> dict:
         """Tiles of the last accumulation per loop (sarbp_plan_tile_paths; synchronises)."""

@@ -114,6 +114,9 @@ struct BpArgs {
     int row_windows;
     // When not NULL: per tile one count added to paths[k], k = the tile's loop (TilePath).
     unsigned *paths;
+    // Grid mode, cubic form: the row coordinate of each pixel in double-float form (an exact
+    // integer part per pulse and column, an f32 cubic per pixel) instead of f64.
+    int range_df;
 };
 // The loop a tile takes (counted in BpArgs::paths).
 enum TilePath {
@@ -122,7 +125,8 @@ enum TilePath {
     kPathPQ = 2,           // range expansion, p, q form
     kPathCubicGather = 3,  // grid mode, cubic form, row elements gathered from HBM
     kPathCubicWindow = 4,  // grid mode, cubic form, row elements from windows in LDS
-    kNumPaths = 5
+    kPathCubicDF = 5,      // grid mode, cubic form, double-float row coordinates
+    kNumPaths = 6
 };
 hipError_t launch_accumulate(const BpArgs &a, hipStream_t stream);
 // Grid mode: whether a plan's or job's expansion tiles take the cubic form, from its whole region

@@ -782,11 +782,23 @@ struct GridCubicB {
 };
 template <int PPT>
 using GridCubic = GridCubicB<kGridBlock<PPT>>;
+// The double-float form of the cubic (range_mode SARBP_RANGE_DF32): per pulse and column the
+// constant term as its exact integer part H = floor(C0 - kRowMagic) and the f32 remainder
+// L = C0 - kRowMagic - H, and C1..C3 in f32, so that a pixel's row coordinate is H plus the f32
+// cubic L + dv (C1 + dv (C2 + dv C3)).
+template <int B>
+struct GridCubicDF {
+    double c[2][B][10];  // per pulse: the c_ij of the f64 form (c_00 + kRowMagic)
+    int h[2][B][16];
+    float l[2][B][16], c1[2][B][16], c2[2][B][16];
+    float c3[2][B];
+};
 template <int PPT>
 struct ExpShared<true, PPT> {
     union {
         double k[2][kExpBlock][6];  // per pulse: T, n.u_hat / lsr, n.v_hat / lsr, -, h, g
         GridCubic<PPT> cub;
+        GridCubicDF<kGridBlock<PPT>> cdf;
     };
 };
 
@@ -830,6 +842,7 @@ struct WinShared {
     union {
         double k[2][kExpBlock][6];
         GridCubicB<kGridBlock<PPT>> cub;  // tiles whose window does not fit
+        GridCubicDF<kGridBlock<PPT>> cdf;  // double-float form
         struct {
             WinCubic wc;
             QPairF32 e[2 * kWinBlock * (kWinSlot<PPT> > 0 ? kWinSlot<PPT> : 1)];
@@ -929,9 +942,8 @@ struct WinTile {
 
 // Grid mode, first stage: c_ij of block blk's pulses, one lane per pulse, in wave blk mod 4.
 static_assert(kBpThreads == 256, "fill_grid_cubic rotates its writer wave over four waves");
-template <int B>
-__device__ inline void fill_grid_cubic(const BpArgs &a, const double *rc, GridCubicB<B> &cub,
-                                       int blk) {
+template <int B, class CUB>
+__device__ inline void fill_grid_cubic(const BpArgs &a, const double *rc, CUB &cub, int blk) {
     const int i = (int)threadIdx.x - 64 * (blk & 3), p = blk * B + i;
     if (i < 0 || i >= B || p >= a.np) return;
     double k[10];
@@ -976,6 +988,26 @@ __device__ inline void fill_grid_columns(const BpArgs &a, GridCubicB<B> &cub, in
         cub.c01[buf][i][x] = make_double2(c0, c1);
         cub.c2[buf][i][x] = __builtin_fma(k[8], du, k[7]);
         if (x == 0) cub.c3[buf][i] = k[9];
+    }
+}
+
+// Double-float form, second stage: the f64 C0..C2 of fill_grid_columns, split and rounded into
+// the GridCubicDF layout (C0 - kRowMagic is exact: both lie in [2^20, 2^21)).
+template <int B>
+__device__ inline void fill_grid_columns_df(const BpArgs &a, GridCubicDF<B> &cub, int blk) {
+    const int n = min(B, a.np - blk * B), buf = blk & 1;
+    for (int job = threadIdx.x; job < 16 * n; job += kBpThreads) {
+        const int i = job >> 4, x = job & 15;
+        const double *k = cub.c[buf][i];
+        const double du = ((double)x - 7.5) * a.g.d_u;
+        const double c0 = __builtin_fma(__builtin_fma(__builtin_fma(k[3], du, k[2]), du, k[1]), du, k[0]);
+        const double c1 = __builtin_fma(__builtin_fma(k[6], du, k[5]), du, k[4]);
+        const double t0 = c0 - kRowMagic, hf = floor(t0);
+        cub.h[buf][i][x] = (int)hf;
+        cub.l[buf][i][x] = (float)(t0 - hf);
+        cub.c1[buf][i][x] = (float)c1;
+        cub.c2[buf][i][x] = (float)__builtin_fma(k[8], du, k[7]);
+        if (x == 0) cub.c3[buf][i] = (float)k[9];
     }
 }
 
@@ -1088,6 +1120,29 @@ __device__ inline void gather_grid(const BpArgs &a, const double (&dv)[PPT], con
     gather_fixed<QT, PPT>(a, y, p, g);
 }
 
+// Double-float form: s = L + dv (C1 + dv (C2 + dv C3)) in f32 is the row coordinate minus the
+// column's integer part H; index H + floor(s), weight fract(s) in the rows' weight unit.
+template <typename QT, int PPT>
+__device__ inline void gather_grid_df(const BpArgs &a, const float (&dv)[PPT], int h, float l,
+                                      float c1, float c2, float c3, int p, Gather<PPT> &g) {
+    constexpr bool kScaled = sizeof(QT) == sizeof(QPairF16);
+    const QT *qbase = reinterpret_cast<const QT *>(a.q);
+    const __amdgpu_buffer_rsrc_t row = row_rsrc(qbase + (size_t)p * a.len, a.len, (int)sizeof(QT));
+    g.qs = kScaled ? a.qscale[p] : 1.0f;
+    int ii[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const float sv =
+            __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c3, dv[j], c2), dv[j], c1), dv[j], l);
+        int fl;
+        asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(fl) : "v"(sv));  // floor(sv)
+        ii[j] = h + fl;  // a negative index reads zeros (buffer bounds check), as in gather_fixed
+        g.ww[j] = __builtin_amdgcn_fractf(sv) * weight_unit<QT>();
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) g.e[j] = loadq(qbase, row, ii[j]);
+}
+
 // The same with the pulse's row window in LDS: y = kRowMagic + (the pixel's element index in the
 // window buffers) + fract, so the high word of y is 0x41380000 + that index, and (hi << 4) + K with
 // K = (LDS address of the buffers) - 0x13800000 (mod 2^32) is the element's LDS address: one
@@ -1157,6 +1212,53 @@ __device__ inline void accumulate_pulses_grid(const BpArgs &a, const double (&dv
             add_terms<QT, PPT, true>(g0, kappa_f, sr, si);
             const int pn = min(p + 2, p1 - 1), i2 = pn - p0;
             gather_grid<QT, PPT>(a, dv, c01[16 * i2], c2[16 * i2], c3[i2], pn, g0);
+            add_terms<QT, PPT, true>(g1, kappa_f, sr, si);
+            if (((p + 2) & (kBpPulseBlock - 1)) == 0) flush();
+        }
+        if ((p1 - p0) & 1) add_terms<QT, PPT, true>(g0, kappa_f, sr, si);  // pulse p1 - 1
+    }
+    flush();
+}
+
+// The loop of accumulate_pulses_grid in the double-float form (same blocks and pipelining).
+template <typename QT, int PPT, int B>
+__device__ inline void accumulate_pulses_grid_df(const BpArgs &a, const float (&dv)[PPT], int col,
+                                                 const double *rc, GridCubicDF<B> &cub,
+                                                 double (&accr)[PPT], double (&acci)[PPT]) {
+    const int np = a.np;
+    const float kappa_f = (float)a.hdr->kappa / weight_unit<QT>();
+    float sr[PPT], si[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) sr[j] = si[j] = 0.f;
+    auto flush = [&] {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            accr[j] += (double)sr[j];
+            acci[j] += (double)si[j];
+            sr[j] = si[j] = 0.f;
+        }
+    };
+    const int nblk = (np + B - 1) / B;
+    fill_grid_cubic<B>(a, rc, cub, 0);
+    if (nblk > 1) fill_grid_cubic<B>(a, rc, cub, 1);
+    __syncthreads();
+    fill_grid_columns_df<B>(a, cub, 0);
+    for (int b = 0; b < nblk; ++b) {
+        __syncthreads();
+        if (b + 2 < nblk) fill_grid_cubic<B>(a, rc, cub, b + 2);
+        if (b + 1 < nblk) fill_grid_columns_df<B>(a, cub, b + 1);
+        const int buf = b & 1, p0 = b * B, p1 = min(np, p0 + B);
+        const int *h = &cub.h[buf][0][col];
+        const float *l = &cub.l[buf][0][col], *c1 = &cub.c1[buf][0][col];
+        const float *c2 = &cub.c2[buf][0][col], *c3 = cub.c3[buf];
+        Gather<PPT> g0, g1;
+        gather_grid_df<QT, PPT>(a, dv, h[0], l[0], c1[0], c2[0], c3[0], p0, g0);
+        for (int p = p0; p + 1 < p1; p += 2) {
+            const int i1 = 16 * (p + 1 - p0);
+            gather_grid_df<QT, PPT>(a, dv, h[i1], l[i1], c1[i1], c2[i1], c3[p + 1 - p0], p + 1, g1);
+            add_terms<QT, PPT, true>(g0, kappa_f, sr, si);
+            const int pn = min(p + 2, p1 - 1), i2 = 16 * (pn - p0);
+            gather_grid_df<QT, PPT>(a, dv, h[i2], l[i2], c1[i2], c2[i2], c3[pn - p0], pn, g0);
             add_terms<QT, PPT, true>(g1, kappa_f, sr, si);
             if (((p + 2) & (kBpPulseBlock - 1)) == 0) flush();
         }
@@ -1475,7 +1577,7 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
     // kernel also needs the largest half-span of its row coordinates over the pulses
     bool expand = false;
     WinTile wt{};
-    const bool want_win = WIN && cubic;
+    const bool want_win = WIN && cubic && !a.range_df;
     if (WIN) {
         wt.U = 7.5 * fabs(a.g.d_u);
         wt.V = (8.0 * PPT - 0.5) * fabs(a.g.d_v);
@@ -1525,6 +1627,7 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
         const int k = path == 0   ? kPathExact
                       : path == 2 ? kPathDirect
                       : !(GRID && cubic) ? kPathPQ
+                      : a.range_df ? kPathCubicDF
                       : wt.W > 0  ? kPathCubicWindow
                                   : kPathCubicGather;
         atomicAdd(&a.paths[k], 1u);
@@ -1537,7 +1640,13 @@ __device__ inline void accumulate_tile_impl(const BpArgs &a, int bx, int by,
             double dv[PPT];
             int col;
             grid_offsets<PPT>(a, bx, by, opaque(tx), opaque(ty), dv, col);
-            if constexpr (WIN) {
+            if (a.range_df) {
+                float dvf[PPT];
+#pragma unroll
+                for (int j = 0; j < PPT; ++j) dvf[j] = (float)dv[j];
+                accumulate_pulses_grid_df<QT, PPT, kGridBlock<PPT>>(a, dvf, col, sh.rc, sh.x.cdf,
+                                                                    accr, acci);
+            } else if constexpr (WIN) {
                 if (wt.W > 0)
                     accumulate_pulses_win<PPT, kWinSlot<PPT>>(a, dv, col, sh.rc, sh.x.w.wc,
                                                               sh.x.w.e, wt, accr, acci);

@@ -92,8 +92,8 @@ int check_desc(const sarbp_plan_desc *d) {
         set_error("fft_blocks %d is negative", d->fft_blocks);
         return halide_error_code_constraint_violated;
     }
-    if (d->range_mode != SARBP_RANGE_F64) {
-        set_error("range_mode %d is not available; use SARBP_RANGE_F64", d->range_mode);
+    if (d->range_mode != SARBP_RANGE_F64 && d->range_mode != SARBP_RANGE_DF32) {
+        set_error("range_mode %d is not SARBP_RANGE_F64 or SARBP_RANGE_DF32", d->range_mode);
         return halide_error_code_constraint_violated;
     }
     return 0;
@@ -422,6 +422,7 @@ extern "C" int sarbp_plan_accumulate(sarbp_plan *p, const void *q_dev, const voi
         a.g = p->grid;
         a.absmax = p->absmax;
         a.paths = p->paths;
+        a.range_df = d.range_mode == SARBP_RANGE_DF32;
         a.row_windows = d.row_windows;
         e = launch_accumulate(a, s);
     }
@@ -510,6 +511,7 @@ int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int6
     a.g = p->grid;
     a.absmax = p->absmax;
     a.paths = p->paths;
+    a.range_df = d.range_mode == SARBP_RANGE_DF32;
     a.row_windows = d.row_windows;
     hipError_t e = launch_accumulate(a, stream);
     if (e != hipSuccess) {

@@ -299,7 +299,8 @@ int check_job(const sarbp_job *j) {
     }
     if (j->norm_r0_policy < SARBP_POLICY_DEFAULT || j->norm_r0_policy > 2 ||
         j->taylor_den_policy < SARBP_POLICY_DEFAULT || j->taylor_den_policy > 1 ||
-        j->q_format < 0 || j->q_format > 1) {
+        j->q_format < 0 || j->q_format > 1 ||
+        (j->range_mode != SARBP_RANGE_F64 && j->range_mode != SARBP_RANGE_DF32)) {
         set_error("sarbp_server_submit: invalid policy or format");
         return halide_error_code_constraint_violated;
     }
@@ -394,6 +395,7 @@ int prepare_job(sarbp_server *s, Job &j, hipStream_t st) {
     a.ppt = j.ppt;
     a.g = grid;
     a.row_windows = j.row_windows;
+    a.range_df = d.range_mode == SARBP_RANGE_DF32;
     j.prepared = true;
     return 0;
 }

@@ -95,8 +95,11 @@ enum sarbp_taylor_den_policy {
 #define SARBP_POLICY_DEFAULT (-1)
 enum sarbp_q_format { SARBP_Q_C32 = 0, SARBP_Q_C16 = 1 };
 enum sarbp_range_mode {
-    SARBP_RANGE_F64 = 0, /* per-pixel f64 |r - pos| (default) */
-    SARBP_RANGE_DF32 = 1 /* double-float (f32 pair) differential range */
+    SARBP_RANGE_F64 = 0, /* each pixel's row coordinate in f64 (default) */
+    SARBP_RANGE_DF32 = 1 /* grid mode, cubic form: each pixel's row coordinate as an exact integer
+                            part per pulse and column plus an f32 cubic in the pixel's offset (a
+                            double-float pair); tiles in other forms, and plans without a grid,
+                            keep f64 */
 };
 
 int sarbp_set_default_policies(int norm_r0_policy, int taylor_den_policy);
@@ -193,7 +196,8 @@ int32_t sarbp_plan_grid_form(const sarbp_plan *p);
 /* How many tiles of the plan's last accumulation (since its pulse table was built) took each loop:
  * counts[0] beyond the rows' cover (clamped, exact phase), [1] direct form, [2] expansion in its
  * p, q form, [3] cubic form gathering row elements from HBM, [4] cubic form reading row windows
- * staged in LDS. Synchronises the device. */
+ * staged in LDS, [5] cubic form with double-float row coordinates (counts has 6 entries).
+ * Synchronises the device. */
 int sarbp_plan_tile_paths(sarbp_plan *p, int64_t *counts);
 
 /* dB and u8 output (img_output_to_dB, img_output_u8) of the image the plan's last accumulation
@@ -286,7 +290,7 @@ typedef struct sarbp_job {
      * (signal.h:40), so the window and the image are non-finite, as the reference's would be; a
      * job stream that wants images at those levels asks for SARBP_TAYLOR_DEN_F32. */
     int32_t norm_r0_policy, taylor_den_policy, q_format;
-    int32_t reserved0;
+    int32_t range_mode;       /* sarbp_range_mode (0: f64) */
     void *out;                /* host complex f64 {2, nu, nv} */
     int32_t reserved[8];
 } sarbp_job;

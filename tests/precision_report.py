@@ -8,7 +8,10 @@ compared with the oracle on
   - windows around the brightest point targets: peak position and peak-sidelobe ratio (PSLR, the
     highest sidelobe outside the 3x3-pixel main lobe relative to the peak, in dB) of the variant
     minus that of the oracle.
-Variants: range profiles stored as complex f32 or complex f16 (per-profile power-of-two scale).
+Variants: range profiles stored as complex f32 (c32) or complex f16 with a per-profile power-of-two
+scale (c16), each with f64 row coordinates; and c32 rows with double-float row coordinates (df32:
+range_mode SARBP_RANGE_DF32, an exact integer part per pulse and column and an f32 cubic per
+pixel).
 
 Usage (on a GPU machine, from the repository root):
     python tests/precision_report.py --config C3 [--sample 2048] [--targets 4]
@@ -48,7 +51,7 @@ def main(argv=None):
     ap.add_argument("--sample", type=int, default=2048)
     ap.add_argument("--targets", type=int, default=4)
     ap.add_argument("--half", type=int, default=16, help="target window half-size (pixels)")
-    ap.add_argument("--variants", default="c32,c16")
+    ap.add_argument("--variants", default="c32,df32,c16")
     args = ap.parse_args(argv)
 
     import torch
@@ -106,8 +109,9 @@ def main(argv=None):
     yi = torch.from_numpy(ys.astype(np.int64)).to(ds.dev)
     xi = torch.from_numpy(xs.astype(np.int64)).to(ds.dev)
     for v in args.variants.split(","):
-        qf = {"c32": sar.Q_C32, "c16": sar.Q_C16}[v]
-        plan = ds.plan(q_format=qf)
+        qf = {"c32": sar.Q_C32, "c16": sar.Q_C16, "df32": sar.Q_C32}[v]
+        rm = sar.RANGE_DF32 if v == "df32" else sar.RANGE_F64
+        plan = ds.plan(q_format=qf, range_mode=rm)
         out = ds.output()
         ds.run(plan, out)
         torch.cuda.synchronize()

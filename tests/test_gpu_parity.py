@@ -427,31 +427,98 @@ def test_row_windows_same_bits(sar, scenes, pipe, torch_dev, case):
         assert paths[0]["cubic_window"] == 0 and paths[0]["cubic_gather"] > 0
 
 
+def test_range_df32_c2_vs_oracle(sar, orc, scenes, pipe, torch_dev):
+    """Double-float row coordinates (range_mode SARBP_RANGE_DF32: per pulse and column an exact
+    integer part, per pixel an f32 cubic in the row offset) on the C2 grid: every tile takes that
+    form, and the image stays within the 1e-4 gate of the FP64 oracle on sampled pixels; its
+    distance from the f64 form is printed."""
+    torch = torch_dev
+    sc = scenes.config_scene("C2")
+    grid = pipe.scene_grid(sc, with_pixel_locs=False)
+    ds = pipe.DeviceScene(sc, grid)
+    nu, nv = grid["nu"], grid["nv"]
+    pdf = ds.plan(range_mode=sar.RANGE_DF32)
+    out = ds.output()
+    ds.run(pdf, out)
+    paths = pdf.tile_paths()
+    ref64 = ds.output()
+    ds.run(ds.plan(), ref64)
+    torch.cuda.synchronize()
+    print(f"DF32 C2 tile paths {paths}")
+    assert paths["cubic_df"] == ((nu + 15) // 16) * ((nv + 63) // 64)
+    a, b = out.cpu().numpy(), ref64.cpu().numpy()
+    assert np.all(np.isfinite(a))
+    d = rel_l2(a, b)
+    rng = np.random.default_rng(29)
+    xs = rng.integers(0, nu, 768)
+    ys = rng.integers(0, nv, 768)
+    phs = ds.phs.cpu().numpy()
+    Q = orc.range_profiles(phs, sc.k_r, sc.taylor, ds.n_fft)
+    _, nr0 = orc.norm_r0(sc.pos)
+    pts = sar.grid_points(grid, xs, ys)
+    cnt = len(xs)
+    ref = orc.backproject_points(Q, sc.nsamples, sc.delta_r, sc.k_r[sc.nsamples // 2], sc.pos,
+                                 nr0, pts, cnt, 1, np.arange(cnt, dtype=np.int32),
+                                 np.zeros(cnt, np.int32), nthreads=NTHREADS)
+    e_df, e_64 = rel_l2(a[ys, xs], ref), rel_l2(b[ys, xs], ref)
+    print(f"DF32 C2: vs f64 form {d:.3e}; sampled rel_l2 vs oracle {e_df:.3e} (f64 form {e_64:.3e})")
+    assert e_df < TOL
+
+
+def test_range_df32_without_grid_is_f64(sar, scenes, pipe, torch_dev):
+    """range_mode SARBP_RANGE_DF32 applies to grid-mode cubic tiles only: a plan reading pixel_locs
+    forms the same image, bit for bit, as its f64 plan (C1)."""
+    torch = torch_dev
+    sc = scenes.config_scene("C1")
+    grid = pipe.scene_grid(sc)
+    ds = pipe.DeviceScene(sc, grid)
+    assert not ds.grid_mode
+    outs = []
+    for mode in (sar.RANGE_F64, sar.RANGE_DF32):
+        out = ds.output()
+        plan = ds.plan(range_mode=mode)
+        ds.run(plan, out)
+        assert plan.tile_paths()["cubic_df"] == 0
+        outs.append(out)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
+
+
 def test_c3_full_size_sampled(sar, orc, scenes, pipe, torch_dev):
     """Configuration C3 at full size (8192^2 x 8000 pulses x 4096 samples, N_fft 16384: the largest
-    FFT held in LDS), pixel locations generated on the device: oracle on 1536 sampled pixels
-    (the four image corners' neighbourhoods, one full row across the image edge to edge, random
-    ones)."""
+    FFT held in LDS), pixel locations generated on the device, in the three precision variants:
+    complex-f32 rows with f64 row coordinates (the plain form), with double-float row coordinates,
+    and complex-f16 rows. Oracle on 1536 sampled pixels (the four image corners' neighbourhoods, one
+    full row across the image edge to edge, random ones). The plain and double-float forms meet the
+    1e-4 gate; f16 rows (11-bit significands) do not (~7e-4) and are held to 2e-3."""
     torch = torch_dev
     sc = scenes.config_scene("C3")
     grid = pipe.scene_grid(sc, with_pixel_locs=False)
     ds = pipe.DeviceScene(sc, grid)
     assert ds.n_fft == 16384
-    plan = ds.plan()
-    out = ds.output()
-    print("C3: running 8192^2 x 8000", flush=True)
-    ds.run(plan, out)
-    torch.cuda.synchronize()
     nu, nv = grid["nu"], grid["nv"]
     rng = np.random.default_rng(11)
     xs = np.concatenate([[0, 1, nu - 1, nu - 2, 0, nu - 1, 0, nu - 1], np.arange(0, nu, 16),
                          rng.integers(0, nu, 1016)])
     ys = np.concatenate([[0, 0, 0, 0, nv - 1, nv - 1, 1, nv - 2], np.full(nu // 16, nv // 3),
                          rng.integers(0, nv, 1016)])
-    got = out[torch.from_numpy(ys).to(out.device), torch.from_numpy(xs).to(out.device)].cpu().numpy()
-    assert np.all(np.isfinite(got))
+    yi, xi = torch.from_numpy(ys).to(ds.dev), torch.from_numpy(xs).to(ds.dev)
+    variants = (("c32 f64", sar.Q_C32, sar.RANGE_F64, TOL),
+                ("c32 df32", sar.Q_C32, sar.RANGE_DF32, TOL),
+                ("c16 f64", sar.Q_C16, sar.RANGE_F64, 2e-3))
+    got = {}
+    for name, qf, rm, _ in variants:
+        plan = ds.plan(q_format=qf, range_mode=rm)
+        out = ds.output()
+        print(f"C3: running 8192^2 x 8000, {name}", flush=True)
+        ds.run(plan, out)
+        torch.cuda.synchronize()
+        got[name] = out[yi, xi].cpu().numpy()
+        assert np.all(np.isfinite(got[name])), name
+        del out, plan
+        torch.cuda.empty_cache()
     phs = ds.phs.cpu().numpy()
-    del out, plan, ds
+    del ds
     Q = orc.range_profiles(phs, sc.k_r, sc.taylor, 16384)
     _, nr0 = orc.norm_r0(sc.pos)
     pts = sar.grid_points(grid, xs, ys)   # a one-row geometry holding just the sampled pixels
@@ -459,9 +526,10 @@ def test_c3_full_size_sampled(sar, orc, scenes, pipe, torch_dev):
     ref = orc.backproject_points(Q, sc.nsamples, sc.delta_r, sc.k_r[sc.nsamples // 2], sc.pos,
                                  nr0, pts, cnt, 1, np.arange(cnt, dtype=np.int32),
                                  np.zeros(cnt, np.int32), nthreads=NTHREADS)
-    e = rel_l2(got, ref)
-    print(f"C3 8192^2 x 8000: sampled rel_l2 {e:.3e} over {cnt} pixels")
-    assert e < TOL
+    for name, _, _, tol in variants:
+        e = rel_l2(got[name], ref)
+        print(f"C3 8192^2 x 8000, {name}: sampled rel_l2 {e:.3e} over {cnt} pixels")
+        assert e < tol, name
 
 
 def test_img_output_parity(sar, orc, torch_dev):

@@ -237,3 +237,29 @@ def test_default_policy_job_equals_plan(sar):
         assert np.array_equal(kept[i], _plan_default_policy(sar, fac, i, recs[i]["ppt"])), i
     f32 = _alone_grid(sar, fac, 1, recs[1]["ppt"])
     assert rel_l2(kept[1], f32) > 1e-3
+
+
+def test_range_df32_jobs_equal_plan(sar):
+    """Jobs in the double-float range form (sarbp_job.range_mode) equal a double-float plan of the
+    same job at the tile shape the engine used, bit for bit, fused and banded."""
+    import torch
+    sv = _serving()
+    specs = [dict(ns=64, npulses=31, side=48, taylor=17), dict(ns=96, npulses=120, side=100, taylor=24),
+             dict(ns=128, npulses=40, side=1500, taylor=21)]
+    fac = sv.JobFactory(specs, seed_base=9800, range_mode=sar.RANGE_DF32)
+    print("serving jobs in the double-float range form", flush=True)
+    with sv.Server(fuse_max_pixel_pulses=FUSE_MAX, autotune=False) as srv:
+        recs, _, kept = sv.replay(srv, fac, None, keep_outputs=range(len(specs)))
+    assert any(r["fused"] for r in recs) and not all(r["fused"] for r in recs)
+    dev = torch.device("cuda", 0)
+    for i in range(len(specs)):
+        sc, g, sp = fac.scenes[i], fac.grids[i], fac.specs[i]
+        plan = sar.Plan(sc.nsamples, sc.npulses, sar.nfft(sc.nsamples, 2), sp["taylor"],
+                        sc.delta_r, sp["side"], sp["side"], taylor_den_policy=fac.taylor_den_policy,
+                        tile_ppt=recs[i]["ppt"], range_mode=sar.RANGE_DF32)
+        plan.set_grid(g["d_u"], g["d_u"], g["u_hat"], g["v_hat"])
+        out = torch.empty((sp["side"], sp["side"]), dtype=torch.complex128, device=dev)
+        plan.execute(torch.from_numpy(np.ascontiguousarray(fac.phs[i])).to(dev),
+                     torch.from_numpy(sc.k_r).to(dev), torch.from_numpy(sc.pos).to(dev), None, out)
+        torch.cuda.synchronize()
+        assert np.array_equal(kept[i], out.cpu().numpy()), i

@@ -165,7 +165,8 @@ hipError_t launch_sim_phs(const double *pos, int np, const double *k64, int ns,
                           const double *tgt, const double *amp, int nt, float2 *phs,
                           hipStream_t stream);
 
-// Device twiddle tables exp(-2 pi i t / N), t in [0, N), cached per (device, N).
+// Device twiddle tables, cached per (device, N): exp(-2 pi i t / N) for t in [0, N), followed by
+// N entries of per-pass copies laid out for the radix-4 passes of the one-row FFT.
 const float2 *twiddles(int device, int n_fft, hipStream_t stream);
 void release_twiddles();
 

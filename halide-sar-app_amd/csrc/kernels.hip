@@ -19,6 +19,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <set>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -246,20 +247,39 @@ __device__ inline void dft_r<4>(float2 *v) {
     v[3] = make_float2(a1.x - a3.x, a1.y - a3.y);
 }
 
+// Twiddles of one radix-4 pass of span Ns over an N-point row: W_N^(r k N / (4 Ns)) for r = 1..3
+// and k in [0, Ns), stored at stage[(r - 1) Ns + k] so that the lanes of a wave (consecutive k)
+// read consecutive entries. The N-point table tw (W_N^t, t in [0, N)) is followed by these stage
+// tables of every radix-4 pass, the pass of span Ns at tw + N + Ns - Ns0 (Ns0 = 2 when log2 N is
+// odd, the radix-2 pass coming first, else 1); their values are entries of tw, so a butterfly
+// rounds alike with either table.
+__device__ inline const float2 *fft_stage_twiddles(const float2 *tw, int N, int logN, int Ns) {
+    return tw + N + Ns - ((logN & 1) ? 2 : 1);
+}
+
 // One Stockham autosort pass of radix R over the LDS-resident row, in place: every thread loads
-// its butterflies' inputs, the block synchronises, then results are written to their sorted slots.
+// its butterflies' inputs and (radix 4) their twiddles, the block synchronises, then results are
+// written to their sorted slots. The twiddle loads do not depend on the row, so they are all issued
+// before the barrier, where their latency overlaps the other waves' reads. stage: this pass's
+// twiddles (fft_stage_twiddles); radix-2 passes run only at Ns = 1, where every twiddle is 1 and
+// none is read. A butterfly with k = 0 takes no twiddle (its factors are 1).
 template <int R, int NB>
-__device__ inline void fft_pass(float2 *buf, int N, int Ns, const float2 *__restrict__ tw,
+__device__ inline void fft_pass(float2 *buf, int N, int Ns, const float2 *__restrict__ stage,
                                 int tid) {
     const int nbf = N / R;
-    const int stride = N / (Ns * R);
     float2 v[NB][R];
+    float2 w[NB][R - 1];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int j = tid + b * kFftThreads;
         if (j < nbf) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * nbf];
+            if constexpr (R == 4) {
+                const int k = j & (Ns - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[b][r - 1] = stage[(r - 1) * Ns + k];
+            }
         }
     }
     __syncthreads();
@@ -268,9 +288,12 @@ __device__ inline void fft_pass(float2 *buf, int N, int Ns, const float2 *__rest
         const int j = tid + b * kFftThreads;
         if (j < nbf) {
             const int k = j & (Ns - 1);
-            if (k != 0) {
+            if constexpr (R == 4) {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * k * stride]);
+                for (int r = 1; r < R; ++r) {
+                    const float2 t = cmul(v[b][r], w[b][r - 1]);
+                    v[b][r] = k != 0 ? t : v[b][r];
+                }
             }
             dft_r<R>(v[b]);
             const int base = (j - k) * R + k;
@@ -290,25 +313,33 @@ __device__ inline int profile_source(int p, int np) {
 
 // Sample x of the FFT input of a pulse whose phase-history row is `row` and whose pulse window is
 // wps: the padded, windowed, fftshifted sample.
+// The filtered, windowed value of phase-history sample a with k_r value kr and range window wsi
+// (pulse window wps).
+__device__ inline float2 windowed_sample(float2 a, float kr, double wsi, double wps) {
+    const float f = fabsf(kr);          // filt = |k_r| (backprojection.cpp:85)
+    const double w = wsi * wps;         // win (backprojection.cpp:82)
+    // phs * filt rounds in f32, then the f64 window applies (backprojection.cpp:90)
+    return make_float2((float)((double)(a.x * f) * w), (float)((double)(a.y * f) * w));
+}
+
 __device__ inline float2 profile_sample(const float2 *__restrict__ row,
                                         const float *__restrict__ k_r,
                                         const double *__restrict__ ws, double wps, int ns, int N,
                                         int xpad, int x) {
     const int xs = (x + N / 2) & (N - 1);  // fftshift (signal_complex.h:24-29)
     const int i = xs - xpad;
-    float2 v = make_float2(0.f, 0.f);
-    if (i >= 0 && i < ns) {
-        const float2 a = row[i];
-        const float f = fabsf(k_r[i]);      // filt = |k_r| (backprojection.cpp:85)
-        const double w = ws[i] * wps;       // win (backprojection.cpp:82)
-        // phs * filt rounds in f32, then the f64 window applies (backprojection.cpp:90)
-        v.x = (float)((double)(a.x * f) * w);
-        v.y = (float)((double)(a.y * f) * w);
-    }
-    return v;
+    if (i >= 0 && i < ns) return windowed_sample(row[i], k_r[i], ws[i], wps);
+    return make_float2(0.f, 0.f);
 }
 
-// Fills buf[0, N) with the padded, windowed, fftshifted samples of pulse s.
+// Fills buf[0, N) with the padded, windowed, fftshifted samples of pulse s: sample i of the pulse
+// lands at xs = i + xpad of the padded row (signal_complex.h:10-21), which fftshift moves to
+// x = (xs + N/2) mod N (signal_complex.h:24-29); every other slot is zero. The slots that hold no
+// sample are cleared first (no loads); then each thread places kRowUnroll samples per trip with
+// all their loads issued before any is used. A lane past the last sample redoes the last one,
+// writing the same value to the same slot as its owner.
+constexpr int kRowUnroll = 4;
+
 template <int T>
 __device__ inline void load_profile_row(float2 *buf, const float2 *__restrict__ phs,
                                         const float *__restrict__ k_r,
@@ -318,7 +349,28 @@ __device__ inline void load_profile_row(float2 *buf, const float2 *__restrict__ 
     const int xpad = floordiv2(N - ns);     // pad's leading offset (signal_complex.h:14)
     const double wps = wp[s];
     const float2 *row = phs + (size_t)s * ns;
-    for (int x = tid; x < N; x += T) buf[x] = profile_sample(row, k_r, ws, wps, ns, N, xpad, x);
+    const int lo = max(xpad, 0), hi = min(xpad + ns, N);  // padded positions holding samples
+    for (int x = tid; x < N; x += T) {
+        const int xs = (x + N / 2) & (N - 1);
+        if (xs < lo || xs >= hi) buf[x] = make_float2(0.f, 0.f);
+    }
+    const int i_lo = lo - xpad, i_hi = hi - xpad;  // samples that land in the row
+    for (int i0 = i_lo + tid; i0 < i_hi; i0 += kRowUnroll * T) {
+        float2 a[kRowUnroll];
+        float kr[kRowUnroll];
+        double w[kRowUnroll];
+        int x[kRowUnroll];
+#pragma unroll
+        for (int u = 0; u < kRowUnroll; ++u) {
+            const int i = min(i0 + u * T, i_hi - 1);
+            x[u] = (i + xpad + N / 2) & (N - 1);
+            a[u] = row[i];
+            kr[u] = k_r[i];
+            w[u] = ws[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kRowUnroll; ++u) buf[x[u]] = windowed_sample(a[u], kr[u], w[u], wps);
+    }
 }
 
 // Writes the transformed row buf[0, N) as profile row p: Q(k) = dft((k + N/2) mod N)
@@ -349,19 +401,37 @@ __device__ inline void store_profile_row(const float2 *buf, float *red, int N,
         scale = ldexpf(1.0f, e);
         if (tid == 0) qscale[p] = ldexpf(1.0f, -e);
     }
-    for (int j = tid; j < len; j += T) {
-        const int k0 = min(max(j - ext, 0), N - 1), k1 = min(max(j - ext + 1, 0), N - 1);
-        const float2 a = buf[(k0 + half) & (N - 1)], b = buf[(k1 + half) & (N - 1)];
-        const float2 q0 = make_float2(a.x * scale, a.y * scale);
-        const float2 dq = make_float2(b.x * scale - q0.x, b.y * scale - q0.y);
-        const float2 r = rot[j];
-        storeq(dst + j, cmul(q0, r), cmul(dq, r));
+    // kStoreUnroll elements per thread and trip: every load of the trip (the rotation and the two
+    // row values) is issued before the first store; stores share the memory counter with loads, so
+    // a load waited for after a store would wait for that store too. A lane past the row's end redoes
+    // its last element, storing the same value to the same place as the element's owner.
+    constexpr int kStoreUnroll = 8;
+    for (int j0 = tid; j0 < len; j0 += kStoreUnroll * T) {
+        float2 a[kStoreUnroll], b[kStoreUnroll], r[kStoreUnroll];
+        int jj[kStoreUnroll];
+#pragma unroll
+        for (int u = 0; u < kStoreUnroll; ++u) {
+            const int j = min(j0 + u * T, len - 1);
+            const int k0 = min(max(j - ext, 0), N - 1), k1 = min(max(j - ext + 1, 0), N - 1);
+            jj[u] = j;
+            a[u] = buf[(k0 + half) & (N - 1)];
+            b[u] = buf[(k1 + half) & (N - 1)];
+            r[u] = rot[j];
+        }
+#pragma unroll
+        for (int u = 0; u < kStoreUnroll; ++u) {
+            const float2 q0 = make_float2(a[u].x * scale, a[u].y * scale);
+            const float2 dq = make_float2(b[u].x * scale - q0.x, b[u].y * scale - q0.y);
+            storeq(dst + jj[u], cmul(q0, r[u]), cmul(dq, r[u]));
+        }
     }
     __syncthreads();
 }
 
-// One block of 512 threads per range profile p, the FFT in place in one LDS row.
-template <typename QT>
+// One block of 512 threads per range profile p, the FFT in place in one LDS row. NB4: radix-4
+// butterflies per thread and pass, N / (4 * kFftThreads) rounded up (4 up to N = 8192, 8 at
+// 16384), so that the registers of smaller rows are sized for them.
+template <typename QT, int NB4>
 __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
     const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
     const double *__restrict__ wp, int ns, int np, int N, int logN,
@@ -375,10 +445,11 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
     __syncthreads();
     int Ns = 1;
     if (logN & 1) {
-        fft_pass<2, kMaxFft / 2 / kFftThreads>(buf, N, 1, tw, tid);
+        fft_pass<2, 2 * NB4>(buf, N, 1, tw, tid);
         Ns = 2;
     }
-    for (; Ns < N; Ns *= 4) fft_pass<4, kMaxFft / 4 / kFftThreads>(buf, N, Ns, tw, tid);
+    for (; Ns < N; Ns *= 4)
+        fft_pass<4, NB4>(buf, N, Ns, fft_stage_twiddles(tw, N, logN, Ns), tid);
     store_profile_row<QT, kFftThreads>(buf, red, N, rot, ext, len, q, qscale, p, tid);
 }
 
@@ -391,17 +462,50 @@ constexpr int kFftSharedThreads = 256;
 
 template <int R>
 __device__ inline void fft_pass_oop(const float2 *src, float2 *dst, int N, int Ns,
-                                    const float2 *__restrict__ tw, int tid) {
+                                    const float2 *__restrict__ stage, int tid) {
     const int nbf = N / R;
-    const int stride = N / (Ns * R);
-    for (int j = tid; j < nbf; j += kFftSharedThreads) {
+    // full trips of kOopGroup butterflies per thread load every input and twiddle of the trip
+    // before the first butterfly; the rest go one at a time
+    constexpr int kOopGroup = 4;
+    int j0 = tid;
+    for (; j0 + (kOopGroup - 1) * kFftSharedThreads < nbf; j0 += kOopGroup * kFftSharedThreads) {
+        float2 v[kOopGroup][R], w[kOopGroup][R - 1];
+#pragma unroll
+        for (int g = 0; g < kOopGroup; ++g) {
+            const int j = j0 + g * kFftSharedThreads;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[g][r] = src[j + r * nbf];
+            if constexpr (R == 4) {
+                const int k = j & (Ns - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[g][r - 1] = stage[(r - 1) * Ns + k];
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < kOopGroup; ++g) {
+            const int j = j0 + g * kFftSharedThreads;
+            const int k = j & (Ns - 1);
+            if constexpr (R == 4) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 t = cmul(v[g][r], w[g][r - 1]);
+                    v[g][r] = k != 0 ? t : v[g][r];
+                }
+            }
+            dft_r<R>(v[g]);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[g][r];
+        }
+    }
+    for (int j = j0; j < nbf; j += kFftSharedThreads) {
         float2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = src[j + r * nbf];
         const int k = j & (Ns - 1);
         if (k != 0) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r * k * stride]);
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], stage[(r - 1) * Ns + k]);
         }
         dft_r<R>(v);
         const int base = (j - k) * R + k;
@@ -434,7 +538,7 @@ __global__ __launch_bounds__(kFftSharedThreads) void range_profile_shared_kernel
             b = t;
         }
         for (; Ns < N; Ns *= 4) {
-            fft_pass_oop<4>(a, b, N, Ns, tw, tid);
+            fft_pass_oop<4>(a, b, N, Ns, fft_stage_twiddles(tw, N, logN, Ns), tid);
             float2 *t = a;
             a = b;
             b = t;
@@ -2035,18 +2139,40 @@ hipError_t launch_range_profiles(const void *phs, const float *k_r, const double
         return hipGetLastError();
     }
     const size_t lds = sizeof(float2) * (size_t)n_fft;
+    static_assert(kMaxFft == 8 * 4 * kFftThreads, "NB4 = 8 covers the largest one-row FFT");
+    const bool big = n_fft > 4 * 4 * kFftThreads;  // NB4 = 8, else 4
+    const void *fn = q_format == SARBP_Q_C16
+                         ? (big ? (const void *)&range_profile_kernel<QPairF16, 8>
+                                : (const void *)&range_profile_kernel<QPairF16, 4>)
+                         : (big ? (const void *)&range_profile_kernel<QPairF32, 8>
+                                : (const void *)&range_profile_kernel<QPairF32, 4>);
+    {
+        static std::mutex m;
+        static std::set<const void *> done;
+        std::lock_guard<std::mutex> lk(m);
+        if (done.insert(fn).second) allow_max_dynamic_lds(fn);
+    }
+    const float2 *rot_c = rot;
     if (q_format == SARBP_Q_C16) {
-        static std::once_flag once;
-        std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_kernel<QPairF16>); });
-        hipLaunchKernelGGL(range_profile_kernel<QPairF16>, dim3(np), dim3(kFftThreads), lds, stream,
-                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
-                           (const float2 *)rot, lay.ext, lay.len, (QPairF16 *)q, qscale);
+        QPairF16 *qq = (QPairF16 *)q;
+        if (big)
+            hipLaunchKernelGGL((range_profile_kernel<QPairF16, 8>), dim3(np), dim3(kFftThreads), lds,
+                               stream, (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
+                               rot_c, lay.ext, lay.len, qq, qscale);
+        else
+            hipLaunchKernelGGL((range_profile_kernel<QPairF16, 4>), dim3(np), dim3(kFftThreads), lds,
+                               stream, (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
+                               rot_c, lay.ext, lay.len, qq, qscale);
     } else {
-        static std::once_flag once;
-        std::call_once(once, [] { allow_max_dynamic_lds((const void *)&range_profile_kernel<QPairF32>); });
-        hipLaunchKernelGGL(range_profile_kernel<QPairF32>, dim3(np), dim3(kFftThreads), lds, stream,
-                           (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
-                           (const float2 *)rot, lay.ext, lay.len, (QPairF32 *)q, (float *)nullptr);
+        QPairF32 *qq = (QPairF32 *)q;
+        if (big)
+            hipLaunchKernelGGL((range_profile_kernel<QPairF32, 8>), dim3(np), dim3(kFftThreads), lds,
+                               stream, (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
+                               rot_c, lay.ext, lay.len, qq, (float *)nullptr);
+        else
+            hipLaunchKernelGGL((range_profile_kernel<QPairF32, 4>), dim3(np), dim3(kFftThreads), lds,
+                               stream, (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
+                               rot_c, lay.ext, lay.len, qq, (float *)nullptr);
     }
     return hipGetLastError();
 }
@@ -2252,14 +2378,26 @@ const float2 *twiddles(int device, int n_fft, hipStream_t stream) {
     auto key = std::make_pair(device, n_fft);
     auto it = g_tw.find(key);
     if (it != g_tw.end()) return it->second;
-    std::vector<float2> h((size_t)n_fft);
+    // [0, N): W_N^t; [N, 2N): the stage tables of the radix-4 passes (fft_stage_twiddles), copies
+    // of entries of the first part
+    const size_t n = (size_t)n_fft;
+    std::vector<float2> h(2 * n, make_float2(0.f, 0.f));
     for (int t = 0; t < n_fft; ++t) {
         const double a = -2.0 * M_PI * (double)t / (double)n_fft;
         h[t] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
+    int logN = 0;
+    while ((1 << logN) < n_fft) ++logN;
+    const int ns0 = (logN & 1) ? 2 : 1;
+    for (int Ns = ns0; Ns < n_fft; Ns *= 4) {
+        const size_t stride = n / (4 * (size_t)Ns);
+        for (int r = 1; r < 4; ++r)
+            for (int k = 0; k < Ns; ++k)
+                h[n + (size_t)(Ns - ns0) + (size_t)(r - 1) * Ns + k] = h[(size_t)r * k * stride];
+    }
     float2 *d = nullptr;
-    if (hipMalloc(&d, sizeof(float2) * (size_t)n_fft) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, h.data(), sizeof(float2) * (size_t)n_fft, hipMemcpyHostToDevice) !=
+    if (hipMalloc(&d, sizeof(float2) * 2 * n) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), sizeof(float2) * 2 * n, hipMemcpyHostToDevice) !=
         hipSuccess) {
         (void)hipFree(d);
         return nullptr;

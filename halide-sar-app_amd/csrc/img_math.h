@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "log_f64.h"
+
 namespace sarbp {
 
 // |z|^2 rounded as the reference rounds it (abs of a ComplexFunc: re * re + im * im, each product
@@ -27,9 +29,14 @@ __device__ inline uint8_t to_u8(double dB, double lo, double hi) {
     return (s == s) ? (uint8_t)s : (uint8_t)0;
 }
 
-// 10 * log10(|z| / m), log10(x) = log(x) / log(Expr(10)) with the f32 log(10) (util.h:56-58)
-__device__ inline double to_db(double2 z, double m, double ln10f) {
-    return 10.0 * (log(sqrt(abs2(z)) / m) / ln10f);
+// 10 * log10(|z| / m), log10(x) = log(x) / log(Expr(10)) with the f32 log(10) (util.h:56-58),
+// formed as log(|z| / m) * db_per_neper with db_per_neper = 10 / (f64) logf(10), rounded once on the
+// host (launch_db_u8): within ~2 ulp of the reference's two roundings, one multiply instead of a
+// division per pixel. |z| / m is the correctly rounded sqrt and division, as the reference forms
+// it; the log is log_f64 (within 1 ulp, a fraction of the device library's instructions), so the
+// brightest pixel is exactly 0 dB.
+__device__ inline double to_db(double2 z, double m, double db_per_neper) {
+    return log_f64(sqrt(abs2(z)) / m) * db_per_neper;
 }
 
 }  // namespace sarbp

@@ -1915,12 +1915,12 @@ __global__ __launch_bounds__(kImgThreads) void absmax_kernel(const double2 *__re
 template <bool VEC>
 __global__ __launch_bounds__(kImgThreads) void db_u8_kernel(
     const double2 *__restrict__ img, int64_t n, const unsigned long long *__restrict__ maxbits,
-    double ln10f, double *__restrict__ dB, uint8_t *__restrict__ u8, double lo, double hi) {
+    double db_per_neper, double *__restrict__ dB, uint8_t *__restrict__ u8, double lo, double hi) {
     const double m = __longlong_as_double((long long)*maxbits);
     if (!VEC) {
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
              i += (int64_t)gridDim.x * blockDim.x) {
-            const double v = to_db(img[i], m, ln10f);
+            const double v = to_db(img[i], m, db_per_neper);
             if (dB) dB[i] = v;
             if (u8) u8[i] = to_u8(v, lo, hi);
         }
@@ -1935,7 +1935,7 @@ __global__ __launch_bounds__(kImgThreads) void db_u8_kernel(
         for (int k = 0; k < 4; ++k) z[k] = img[4 * q + k];
         double v[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = to_db(z[k], m, ln10f);
+        for (int k = 0; k < 4; ++k) v[k] = to_db(z[k], m, db_per_neper);
         if (dB) {
             double2 *d2 = reinterpret_cast<double2 *>(dB + 4 * q);
             d2[0] = make_double2(v[0], v[1]);
@@ -1950,7 +1950,7 @@ __global__ __launch_bounds__(kImgThreads) void db_u8_kernel(
     }
     const int64_t i = 4 * n4 + t0;
     if (i < n) {
-        const double v = to_db(img[i], m, ln10f);
+        const double v = to_db(img[i], m, db_per_neper);
         if (dB) dB[i] = v;
         if (u8) u8[i] = to_u8(v, lo, hi);
     }
@@ -2321,14 +2321,15 @@ hipError_t launch_db_u8(const double2 *img, int64_t npix, const unsigned long lo
     // 4 pixels per thread; the tail (npix mod 4 < 4 pixels) is taken by the first threads
     const int64_t want = (npix / 4 + kImgThreads - 1) / kImgThreads;
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, 4096));
-    const double ln10f = (double)std::log(10.0f);
+    // 10 / log(10) with the f32 log(10) of the reference (util.h:56-58), rounded once
+    const double db_per_neper = 10.0 / (double)std::log(10.0f);
     const bool vec = (uintptr_t)dB % 16 == 0 && (uintptr_t)u8 % 4 == 0;
     if (vec)
         hipLaunchKernelGGL(db_u8_kernel<true>, dim3(blocks), dim3(kImgThreads), 0, stream, img,
-                           npix, maxbits, ln10f, dB, u8, lo, hi);
+                           npix, maxbits, db_per_neper, dB, u8, lo, hi);
     else
         hipLaunchKernelGGL(db_u8_kernel<false>, dim3(blocks), dim3(kImgThreads), 0, stream, img,
-                           npix, maxbits, ln10f, dB, u8, lo, hi);
+                           npix, maxbits, db_per_neper, dB, u8, lo, hi);
     return hipGetLastError();
 }
 

@@ -237,8 +237,10 @@ def main():
                "cores": nthreads, "kind": "port",
                "code": ("oracle/sarbp_oracle.c: the reference CPU schedule's loop nest "
                         "(backprojection.cpp:302-314: rows serial, 4-pixel groups over threads, "
-                        "norm_rr0[4 x np] staged per group) in scalar f64 C with libm sin/cos; "
-                        "not vectorised"),
+                        "norm_rr0[4 x np] staged per group) in scalar f64 C with libm sin/cos. "
+                        "The reference vectorises norm_rr0 over pulses and the output over x; "
+                        "its accumulation update (img.inner.update, :306-307) is not vectorised, "
+                        "and runs scalar here as there"),
                "usable_cores": cpu_info["usable"], "cpu_model": cpu_info["model"],
                "sample": (f"{rows} of {grid['nv']} rows ({rows * grid['nu']} pixels) x "
                           f"{sc.npulses} pulses, median of 3 runs {t_rows:.2f} s "

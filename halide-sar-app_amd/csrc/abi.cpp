@@ -226,11 +226,7 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
         set_error("output_img dimension 0 must lie within [0, 2) (real, imaginary)");
         return halide_error_code_access_out_of_bounds;
     }
-    if (N_fft > kMaxFftTwoPass || N_fft < 2 || (N_fft & (N_fft - 1))) {
-        set_error("N_fft (%d) must be a power of two in [2, %d]", N_fft, kMaxFftTwoPass);
-        return N_fft > kMaxFftTwoPass ? halide_error_code_param_too_large
-                                      : halide_error_code_param_too_small;
-    }
+    if (int rv = check_n_fft(N_fft)) return rv;
     const int ox0 = out->dim[1].min, oy0 = out->dim[2].min;
     const int ow = out->dim[1].extent, oh = out->dim[2].extent;
     if (ow == 0 || oh == 0 || out->dim[0].extent == 0) return 0;
@@ -637,7 +633,7 @@ extern "C" int img_output_u8(halide_buffer_t *dB, double dB_min, double dB_max,
 namespace sarbp {
 // dft.cpp:20-30 builds the process-wide FFTW plan; here it builds the device twiddle table.
 void abi_dft_init(size_t N_fft) {
-    if (N_fft < 2 || N_fft > (size_t)kMaxFftTwoPass || (N_fft & (N_fft - 1))) return;
+    if (N_fft > (size_t)kMaxFftTwoPass || check_n_fft((int)N_fft) != 0) return;
     if (sarbp_device_count() <= 0) return;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return;

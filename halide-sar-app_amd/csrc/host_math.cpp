@@ -33,6 +33,23 @@ void set_error(const char *fmt, ...) {
 
 const char *last_error() { return g_last_error.c_str(); }
 
+// dft.cpp:20-30 plans FFTW for any N_fft; here powers of two run the LDS (N <= kMaxFft) or
+// four-step (N <= kMaxFftTwoPass) FFT and other lengths Bluestein's convolution (N <=
+// kMaxFftBluestein).
+int check_n_fft(int n_fft) {
+    const bool pow2 = n_fft > 0 && (n_fft & (n_fft - 1)) == 0;
+    if (n_fft < 2) {
+        set_error("N_fft (%d) must be at least 2", n_fft);
+        return halide_error_code_param_too_small;
+    }
+    if (pow2 ? n_fft > kMaxFftTwoPass : n_fft > kMaxFftBluestein) {
+        set_error("N_fft (%d) must be a power of two up to %d or another length up to %d", n_fft,
+                  kMaxFftTwoPass, kMaxFftBluestein);
+        return halide_error_code_param_too_large;
+    }
+    return 0;
+}
+
 // backprojection.cpp:120 and :53 — the interpolation axis is linspace(floor(-ns dr / 2),
 // floor(ns dr / 2), N); rows extend it by N / 4 elements on each side.
 RowLayout row_layout(int ns, int n_fft, double delta_r) {

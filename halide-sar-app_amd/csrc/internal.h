@@ -165,9 +165,15 @@ hipError_t launch_sim_phs(const double *pos, int np, const double *k64, int ns,
                           const double *tgt, const double *amp, int nt, float2 *phs,
                           hipStream_t stream);
 
-// Device twiddle tables, cached per (device, N): exp(-2 pi i t / N) for t in [0, N), followed by
-// N entries of per-pass copies laid out for the radix-4 passes of the one-row FFT.
+// Device twiddle tables, cached per (device, N). A power of two N: exp(-2 pi i t / N) for t in
+// [0, N), followed by N entries of per-pass copies laid out for the radix-4 passes of the one-row
+// FFT. Any other N (<= kMaxFftBluestein): the tables of range_profile_bluestein_kernel, whose FFT
+// length is M = 2^bluestein_log_m(N) >= 2N - 1.
 const float2 *twiddles(int device, int n_fft, hipStream_t stream);
+int bluestein_log_m(int n_fft);
+// 0 if the pipeline takes this N_fft (a power of two in [2, kMaxFftTwoPass], or any other value
+// in [3, kMaxFftBluestein]); else sets the error message and returns param_too_small/large.
+int check_n_fft(int n_fft);
 void release_twiddles();
 
 // Longest FFT done in one LDS row (one workgroup per profile), and the longest done in two passes
@@ -175,6 +181,9 @@ void release_twiddles();
 // coordinate of the accumulation).
 constexpr int kMaxFft = 16384;
 constexpr int kMaxFftTwoPass = 131072;
+// Largest N_fft that is not a power of two: Bluestein's convolution length 2^ceil(log2(2N - 1))
+// must fit one LDS row (kMaxFft).
+constexpr int kMaxFftBluestein = kMaxFft / 2;
 
 // Plan steps for callers that split the accumulation into row bands (the Halide-buffer entry point
 // overlaps transfers with computation this way). plan_prepare enqueues the range profiles and the

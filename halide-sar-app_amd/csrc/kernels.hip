@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <map>
 #include <mutex>
 #include <set>
@@ -373,16 +374,17 @@ __device__ inline void load_profile_row(float2 *buf, const float2 *__restrict__ 
     }
 }
 
-// Writes the transformed row buf[0, N) as profile row p: Q(k) = dft((k + N/2) mod N)
-// (backprojection.cpp:104) as rotated (A, B) elements; with qscale (f16 storage) the row is first
-// scaled by a power of two. `red` holds T / 64 floats. Ends with a barrier.
+// Writes the transformed row buf[0, N) as profile row p: Q(k) = dft((k + mid) mod N) with
+// mid = N/2 + N%2 (fftshift, signal_complex.h:24-29; backprojection.cpp:104) as rotated (A, B)
+// elements; with qscale (f16 storage) the row is first scaled by a power of two. `red` holds T / 64
+// floats. Ends with a barrier.
 template <typename QT, int T>
 __device__ inline void store_profile_row(const float2 *buf, float *red, int N,
                                          const float2 *__restrict__ rot, int ext, int len,
                                          QT *__restrict__ q, float *__restrict__ qscale, int p,
                                          int tid) {
 #pragma clang fp contract(off)
-    const int half = N / 2;
+    const int mid = N / 2 + N % 2;
     QT *dst = q + (size_t)p * len;
     float scale = 1.0f;
     if (qscale) {
@@ -414,8 +416,9 @@ __device__ inline void store_profile_row(const float2 *buf, float *red, int N,
             const int j = min(j0 + u * T, len - 1);
             const int k0 = min(max(j - ext, 0), N - 1), k1 = min(max(j - ext + 1, 0), N - 1);
             jj[u] = j;
-            a[u] = buf[(k0 + half) & (N - 1)];
-            b[u] = buf[(k1 + half) & (N - 1)];
+            const int s0 = k0 + mid, s1 = k1 + mid;  // < 2N
+            a[u] = buf[s0 >= N ? s0 - N : s0];
+            b[u] = buf[s1 >= N ? s1 - N : s1];
             r[u] = rot[j];
         }
 #pragma unroll
@@ -450,6 +453,66 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
     }
     for (; Ns < N; Ns *= 4)
         fft_pass<4, NB4>(buf, N, Ns, fft_stage_twiddles(tw, N, logN, Ns), tid);
+    store_profile_row<QT, kFftThreads>(buf, red, N, rot, ext, len, q, qscale, p, tid);
+}
+
+// Range profiles of an N_fft that is not a power of two (N <= kMaxFftBluestein), by Bluestein's
+// chirp-z identity nk = (n^2 + k^2 - (k - n)^2) / 2:
+//   X_k = w_k sum_n (x_n w_n) conj(w_(k-n)),   w_n = exp(-i pi n^2 / N)
+// a circular convolution of length M = 2^ceil(log2(2N - 1)) computed with two M-point FFTs in LDS:
+//   buf = FFT_M(a), a_n = x_n w_n (n < N, else 0)
+//   buf = FFT_M(conj(buf * Bs)),  Bs = FFT_M(b) / M,  b_m = conj(w_|m|) for |m| < N (mod M)
+//   X_k = w_k conj(buf_k)                     (the inverse FFT as conj(FFT(conj(.))) / M)
+// then the row store of the power-of-two kernels. x_n is the padded, windowed, fftshifted sample
+// (fftshift by mid = N/2 + N%2, signal_complex.h:24-29). Tables (bluestein_tables, after the
+// twiddles of M in the same allocation): Bs at tw + 2M, w at tw + 3M.
+template <typename QT, int NB4>
+__global__ __launch_bounds__(kFftThreads) void range_profile_bluestein_kernel(
+    const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
+    const double *__restrict__ wp, int ns, int np, int N, int logM,
+    const float2 *__restrict__ tw, const float2 *__restrict__ rot, int ext, int len,
+    QT *__restrict__ q, float *__restrict__ qscale) {
+    extern __shared__ float2 buf[];
+    __shared__ float red[kFftThreads / 64];
+    const int M = 1 << logM;
+    const float2 *Bs = tw + 2 * (size_t)M;
+    const float2 *w = tw + 3 * (size_t)M;
+    const int p = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int s = profile_source(p, np);
+    const int xpad = floordiv2(N - ns);
+    const double wps = wp[s];
+    const float2 *row = phs + (size_t)s * ns;
+    const int mid = N / 2 + N % 2;
+    for (int n = tid; n < M; n += kFftThreads) {
+        float2 v = make_float2(0.f, 0.f);
+        if (n < N) {
+            const int xs = n + mid >= N ? n + mid - N : n + mid;
+            const int i = xs - xpad;
+            if (i >= 0 && i < ns) v = cmul(windowed_sample(row[i], k_r[i], ws[i], wps), w[n]);
+        }
+        buf[n] = v;
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+        int Ns = 1;
+        if (logM & 1) {
+            fft_pass<2, 2 * NB4>(buf, M, 1, tw, tid);
+            Ns = 2;
+        }
+        for (; Ns < M; Ns *= 4)
+            fft_pass<4, NB4>(buf, M, Ns, fft_stage_twiddles(tw, M, logM, Ns), tid);
+        for (int k = tid; k < M; k += kFftThreads) {
+            const float2 y = buf[k];
+            if (pass == 0) {
+                const float2 t = cmul(y, Bs[k]);
+                buf[k] = make_float2(t.x, -t.y);
+            } else if (k < N) {
+                buf[k] = cmul(w[k], make_float2(y.x, -y.y));
+            }
+        }
+        __syncthreads();
+    }
     store_profile_row<QT, kFftThreads>(buf, red, N, rot, ext, len, q, qscale, p, tid);
 }
 
@@ -2109,6 +2172,48 @@ hipError_t launch_range_profiles(const void *phs, const float *k_r, const double
     while ((1 << logN) < n_fft) ++logN;
     hipLaunchKernelGGL(rotation_kernel, dim3((lay.len + 255) / 256), dim3(256), 0, stream, k_r, ns,
                        lay.lsa, lay.lsr, lay.ext, lay.len, rot);
+    if ((n_fft & (n_fft - 1)) != 0) {
+        if (n_fft > kMaxFftBluestein) return hipErrorInvalidValue;
+        const int logM = bluestein_log_m(n_fft);
+        const size_t lds = sizeof(float2) * ((size_t)1 << logM);
+        const bool big = logM > 13;  // NB4 = 8 for M = 16384
+        const void *fn = q_format == SARBP_Q_C16
+                             ? (big ? (const void *)&range_profile_bluestein_kernel<QPairF16, 8>
+                                    : (const void *)&range_profile_bluestein_kernel<QPairF16, 4>)
+                             : (big ? (const void *)&range_profile_bluestein_kernel<QPairF32, 8>
+                                    : (const void *)&range_profile_bluestein_kernel<QPairF32, 4>);
+        {
+            static std::mutex m;
+            static std::set<const void *> done;
+            std::lock_guard<std::mutex> lk(m);
+            if (done.insert(fn).second) allow_max_dynamic_lds(fn);
+        }
+        const float2 *rot_c = rot;
+        if (q_format == SARBP_Q_C16) {
+            if (big)
+                hipLaunchKernelGGL((range_profile_bluestein_kernel<QPairF16, 8>), dim3(np),
+                                   dim3(kFftThreads), lds, stream, (const float2 *)phs, k_r, ws, wp,
+                                   ns, np, n_fft, logM, tw, rot_c, lay.ext, lay.len,
+                                   (QPairF16 *)q, qscale);
+            else
+                hipLaunchKernelGGL((range_profile_bluestein_kernel<QPairF16, 4>), dim3(np),
+                                   dim3(kFftThreads), lds, stream, (const float2 *)phs, k_r, ws, wp,
+                                   ns, np, n_fft, logM, tw, rot_c, lay.ext, lay.len,
+                                   (QPairF16 *)q, qscale);
+        } else {
+            if (big)
+                hipLaunchKernelGGL((range_profile_bluestein_kernel<QPairF32, 8>), dim3(np),
+                                   dim3(kFftThreads), lds, stream, (const float2 *)phs, k_r, ws, wp,
+                                   ns, np, n_fft, logM, tw, rot_c, lay.ext, lay.len,
+                                   (QPairF32 *)q, (float *)nullptr);
+            else
+                hipLaunchKernelGGL((range_profile_bluestein_kernel<QPairF32, 4>), dim3(np),
+                                   dim3(kFftThreads), lds, stream, (const float2 *)phs, k_r, ws, wp,
+                                   ns, np, n_fft, logM, tw, rot_c, lay.ext, lay.len,
+                                   (QPairF32 *)q, (float *)nullptr);
+        }
+        return hipGetLastError();
+    }
     if (n_fft > kMaxFft) {
         if (n_fft > kMaxFftTwoPass || !scratch) return hipErrorInvalidValue;
         if (q_format == SARBP_Q_C16)
@@ -2374,32 +2479,101 @@ std::mutex g_tw_mu;
 std::map<std::pair<int, int>, float2 *> g_tw;
 }  // namespace
 
+namespace {
+
+// Stage tables of the radix-4 passes of an n-point FFT (fft_stage_twiddles), written to
+// h[n, 2n) from the n-point table in h[0, n).
+void fill_stage_twiddles(std::vector<float2> &h, size_t base, int n) {
+    int logN = 0;
+    while ((1 << logN) < n) ++logN;
+    const int ns0 = (logN & 1) ? 2 : 1;
+    for (int Ns = ns0; Ns < n; Ns *= 4) {
+        const size_t stride = (size_t)n / (4 * (size_t)Ns);
+        for (int r = 1; r < 4; ++r)
+            for (int k = 0; k < Ns; ++k)
+                h[base + n + (size_t)(Ns - ns0) + (size_t)(r - 1) * Ns + k] =
+                    h[base + (size_t)r * k * stride];
+    }
+}
+
+// In-place forward FFT of a power-of-two length in f64 (iterative radix 2, twiddles evaluated
+// directly), for the host-side Bluestein table.
+void fft_f64_host(std::vector<std::complex<double>> &a) {
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const size_t half = len / 2;
+        for (size_t k = 0; k < half; ++k) {
+            const double ang = -2.0 * M_PI * (double)k / (double)len;
+            const std::complex<double> wk(std::cos(ang), std::sin(ang));
+            for (size_t i = 0; i < n; i += len) {
+                const std::complex<double> u = a[i + k], v = a[i + k + half] * wk;
+                a[i + k] = u + v;
+                a[i + k + half] = u - v;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+int bluestein_log_m(int n_fft) {
+    int logM = 0;
+    while ((1 << logM) < 2 * n_fft - 1) ++logM;
+    return logM;
+}
+
 const float2 *twiddles(int device, int n_fft, hipStream_t stream) {
     std::lock_guard<std::mutex> lk(g_tw_mu);
     auto key = std::make_pair(device, n_fft);
     auto it = g_tw.find(key);
     if (it != g_tw.end()) return it->second;
-    // [0, N): W_N^t; [N, 2N): the stage tables of the radix-4 passes (fft_stage_twiddles), copies
-    // of entries of the first part
-    const size_t n = (size_t)n_fft;
-    std::vector<float2> h(2 * n, make_float2(0.f, 0.f));
-    for (int t = 0; t < n_fft; ++t) {
-        const double a = -2.0 * M_PI * (double)t / (double)n_fft;
-        h[t] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-    int logN = 0;
-    while ((1 << logN) < n_fft) ++logN;
-    const int ns0 = (logN & 1) ? 2 : 1;
-    for (int Ns = ns0; Ns < n_fft; Ns *= 4) {
-        const size_t stride = n / (4 * (size_t)Ns);
-        for (int r = 1; r < 4; ++r)
-            for (int k = 0; k < Ns; ++k)
-                h[n + (size_t)(Ns - ns0) + (size_t)(r - 1) * Ns + k] = h[(size_t)r * k * stride];
+    std::vector<float2> h;
+    if ((n_fft & (n_fft - 1)) == 0) {
+        // [0, N): W_N^t; [N, 2N): the stage tables of the radix-4 passes (fft_stage_twiddles),
+        // copies of entries of the first part
+        const size_t n = (size_t)n_fft;
+        h.assign(2 * n, make_float2(0.f, 0.f));
+        for (int t = 0; t < n_fft; ++t) {
+            const double a = -2.0 * M_PI * (double)t / (double)n_fft;
+            h[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        fill_stage_twiddles(h, 0, n_fft);
+    } else {
+        // Bluestein tables of range_profile_bluestein_kernel, M = 2^ceil(log2(2N - 1)):
+        // [0, M) W_M^t, [M, 2M) the stage tables of M, [2M, 3M) FFT_M(b) / M, [3M, 3M + N) the chirp
+        // w_n = exp(-i pi n^2 / N) (n^2 reduced mod 2N in integers), b_m = conj(w_|m|) for |m| < N
+        const int M = 1 << bluestein_log_m(n_fft);
+        h.assign(3 * (size_t)M + (size_t)n_fft, make_float2(0.f, 0.f));
+        for (int t = 0; t < M; ++t) {
+            const double a = -2.0 * M_PI * (double)t / (double)M;
+            h[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        fill_stage_twiddles(h, 0, M);
+        std::vector<std::complex<double>> w((size_t)n_fft), b((size_t)M);
+        for (int m = 0; m < n_fft; ++m) {
+            const long long r = ((long long)m * m) % (2LL * n_fft);
+            const double a = -M_PI * (double)r / (double)n_fft;
+            w[m] = std::complex<double>(std::cos(a), std::sin(a));
+        }
+        for (int m = 0; m < n_fft; ++m) {
+            b[m] = std::conj(w[m]);
+            if (m > 0) b[(size_t)M - m] = std::conj(w[m]);
+        }
+        fft_f64_host(b);
+        for (int k = 0; k < M; ++k)
+            h[2 * (size_t)M + k] = make_float2((float)(b[k].real() / M), (float)(b[k].imag() / M));
+        for (int m = 0; m < n_fft; ++m)
+            h[3 * (size_t)M + m] = make_float2((float)w[m].real(), (float)w[m].imag());
     }
     float2 *d = nullptr;
-    if (hipMalloc(&d, sizeof(float2) * 2 * n) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, h.data(), sizeof(float2) * 2 * n, hipMemcpyHostToDevice) !=
-        hipSuccess) {
+    if (hipMalloc(&d, sizeof(float2) * h.size()) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(d);
         return nullptr;
     }

@@ -36,7 +36,6 @@ std::mutex g_policy_mu;
 int g_norm_policy = SARBP_NORM_R0_F32_STRICT;
 int g_den_policy = SARBP_TAYLOR_DEN_INT;
 
-bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 struct DeviceGuard {
     int prev = -1;
@@ -61,11 +60,7 @@ int check_desc(const sarbp_plan_desc *d) {
                   d->nsamples, d->npulses);
         return halide_error_code_param_too_small;
     }
-    if (!is_pow2(d->n_fft) || d->n_fft < 2 || d->n_fft > kMaxFftTwoPass) {
-        set_error("N_fft (%d) must be a power of two in [2, %d]", d->n_fft, kMaxFftTwoPass);
-        return d->n_fft > kMaxFftTwoPass ? halide_error_code_param_too_large
-                                         : halide_error_code_param_too_small;
-    }
+    if (int rv = check_n_fft(d->n_fft)) return rv;
     if (d->nu < 1 || d->nv < 1 || d->width < 0 || d->height < 0 || d->r_extent < 0) {
         set_error("image extents must be positive and region extents non-negative");
         return halide_error_code_buffer_extents_negative;

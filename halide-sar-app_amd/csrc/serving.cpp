@@ -290,7 +290,8 @@ int check_job(const sarbp_job *j) {
     }
     if (j->n_fft < 2 || j->n_fft > kMaxFft || (j->n_fft & (j->n_fft - 1))) {
         set_error("sarbp_server_submit: N_fft (%d) must be a power of two in [2, %d] (longer "
-                  "profiles run through plans and backprojection())", j->n_fft, kMaxFft);
+                  "profiles and other lengths run through plans and backprojection())",
+                  j->n_fft, kMaxFft);
         return halide_error_code_param_too_large;
     }
     if ((int64_t)j->nu * j->nv > INT32_MAX) {

@@ -38,8 +38,9 @@ extern "C" {
  *   phs        f32 {2, nsamples, npulses}          (PlatformData.cpp:238)
  *   k_r        f32 {nsamples}                      (PlatformData.cpp:137)
  *   taylor_s_l Taylor sidelobe level in dB         (backprojection.cpp:37)
- *   N_fft      FFT length (power of two, 2..131072; above 16384 a four-step FFT through
- *              HBM) (backprojection.cpp:38, sarbp.cpp:258)
+ *   N_fft      FFT length (backprojection.cpp:38, sarbp.cpp:258): a power of two 2..131072
+ *              (above 16384 a four-step FFT through HBM) or any other length 3..8192
+ *              (Bluestein's chirp convolution); larger values return param_too_large
  *   delta_r    range resolution                    (backprojection.cpp:39)
  *   u, v       f64 {nu}, {nv}; only extents used   (backprojection.cpp:71-72)
  *   pos        f32 {3, npulses}                    (PlatformData.cpp:219)

@@ -131,23 +131,36 @@ static int floordiv_i(int a, int b) {
 int orc_fft_f64(double *data, int n) {
     if (n <= 0) return -1;
     if (!is_pow2(n)) {
-        /* direct definition X_k = sum_j x_j exp(-2 pi i j k / n) */
+        /* direct definition X_k = sum_j x_j exp(-2 pi i j k / n), the exponent reduced to
+           (j k) mod n and its cos / sin evaluated once per residue */
         double *tmp = (double *)malloc(sizeof(double) * 2 * (size_t)n);
-        if (!tmp) return -1;
+        double *cs = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+        if (!tmp || !cs) {
+            free(tmp);
+            free(cs);
+            return -1;
+        }
+        for (int m = 0; m < n; m++) {
+            double a = -2.0 * M_PI * (double)m / (double)n;
+            cs[2 * m] = cos(a);
+            cs[2 * m + 1] = sin(a);
+        }
         for (int k = 0; k < n; k++) {
             double sr = 0.0, si = 0.0;
+            long long jk = 0; /* (j k) mod n, advanced by k per step */
             for (int j = 0; j < n; j++) {
-                long long jk = ((long long)j * k) % n;
-                double a = -2.0 * M_PI * (double)jk / (double)n;
-                double c = cos(a), s = sin(a);
+                double c = cs[2 * jk], s = cs[2 * jk + 1];
                 sr += data[2 * j] * c - data[2 * j + 1] * s;
                 si += data[2 * j] * s + data[2 * j + 1] * c;
+                jk += k;
+                if (jk >= n) jk -= n;
             }
             tmp[2 * k] = sr;
             tmp[2 * k + 1] = si;
         }
         memcpy(data, tmp, sizeof(double) * 2 * (size_t)n);
         free(tmp);
+        free(cs);
         return 0;
     }
     /* iterative radix-2 decimation in time with bit-reversed input order */

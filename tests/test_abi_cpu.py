@@ -74,8 +74,10 @@ def test_validation_error_codes(sar):
     # u float32
     rv = sar.backprojection(phs, k_r, 17, 64, 0.2, u.astype(np.float32), v, pos, r, out)
     assert rv == E["bad_type"]
-    # N_fft not a power of two / too large / too small
-    assert sar.backprojection(phs, k_r, 17, 100, 0.2, u, v, pos, r, out) == E["param_too_small"]
+    # N_fft: powers of two up to 2^17 and other lengths up to 8192 are taken (dft.cpp:20-30 plans
+    # FFTW for any N); below 2, other lengths above 8192 and powers of two above 2^17 are not
+    assert sar.backprojection(phs, k_r, 17, 1, 0.2, u, v, pos, r, out) == E["param_too_small"]
+    assert sar.backprojection(phs, k_r, 17, 8193, 0.2, u, v, pos, r, out) == E["param_too_large"]
     assert sar.backprojection(phs, k_r, 17, 1 << 18, 0.2, u, v, pos, r, out) == E["param_too_large"]
     # k_r shorter than nsamples: access out of bounds
     assert sar.backprojection(phs, k_r[:10], 17, 64, 0.2, u, v, pos, r, out) == \
@@ -103,6 +105,7 @@ def test_bounds_query_fills_required_regions(sar):
     rq = sar.Buffer.bounds_query([(0, 0, 0), (0, 0, 0)], F, 64)
     rv = sar.backprojection(phs, kq, 17, 64, 0.2, u, v, pq, rq, out)
     assert rv == 0
+    assert sar.backprojection(phs, kq, 17, 100, 0.2, u, v, pq, rq, out) == 0  # any length N_fft
     assert kq.dims() == [(0, 24, 1)]
     assert pq.dims() == [(0, 3, 1), (0, 16, 3)]
     assert rq.dims() == [(0, 32 * 16, 1), (0, 3, 32 * 16)]
@@ -116,6 +119,9 @@ def test_no_gpu_is_reported_not_aborted(sar):
     rv = sar.backprojection(phs, k_r, 17, 64, 0.2, u, v, pos, r, out)
     assert rv == sar.ERR["no_device_interface"]
     assert "no HIP device" in sar.last_error()
+    # a length that is not a power of two passes validation and reaches the device the same way
+    rv = sar.backprojection(phs, k_r, 17, 100, 0.2, u, v, pos, r, out)
+    assert rv == sar.ERR["no_device_interface"]
 
 
 def test_policy_defaults_and_validation(sar):

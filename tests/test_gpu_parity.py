@@ -186,6 +186,55 @@ def test_range_profiles_four_step(sar, orc, scenes, torch_dev, ns, np_, q_format
     assert err < (1e-5 if q_format == 0 else 2e-3)
 
 
+@pytest.mark.parametrize("ns,np_,n_fft,q_format", [(24, 16, 45, 0), (300, 37, 1000, 0),
+                                                    (600, 9, 1536, 1), (1000, 8, 3000, 0),
+                                                    (2048, 7, 4097, 0), (4000, 2, 8191, 0),
+                                                    (5000, 3, 3001, 0)])
+def test_range_profiles_bluestein_vs_oracle(sar, orc, scenes, torch_dev, ns, np_, n_fft, q_format):
+    """N_fft that is not a power of two (FFTW plans any N, dft.cpp:20-30): Bluestein's chirp
+    convolution through two power-of-two FFTs of M >= 2N - 1 points in LDS. Odd and even lengths,
+    N = 45 (M 128) to 8191 (M 16384, the largest), an odd pulse count and both row formats, and
+    N shorter than the samples (negative pad, ns 5000 > N 3001). Q against the oracle's direct
+    FP64 DFT with the reference's fftshift by ceil(N / 2)."""
+    torch = torch_dev
+    assert n_fft & (n_fft - 1)
+    sc = scenes.tiny_scene(npulses=np_, nsamples=ns, seed=ns + n_fft)
+    plan = sar.Plan(sc.nsamples, sc.npulses, n_fft, sc.taylor, sc.delta_r, 8, 8, q_format=q_format)
+    print(f"Bluestein FFT: ns={ns} np={np_} N={n_fft} format={q_format}", flush=True)
+    plan.range_profiles(torch.from_numpy(sc.phs).cuda(), torch.from_numpy(sc.k_r).cuda())
+    got = plan.read_profiles()
+    ref = orc.range_profiles(sc.phs, sc.k_r, sc.taylor, n_fft)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"   Q vs oracle: max err / max |Q| = {err:.3e}")
+    assert err < (1e-5 if q_format == 0 else 2e-3)
+
+
+def test_backprojection_n_fft_not_power_of_two_vs_oracle(sar, orc, scenes, torch_dev):
+    """backprojection() with N_fft 1500 (not a power of two) through the Halide-buffer entry point:
+    the Bluestein range profiles and an interpolation axis of 1500 points, the whole 48 x 48 image
+    against the oracle; 8193 and other lengths above 8192 are refused."""
+    sc = scenes.circular_arc_scene(npulses=96, nsamples=424, f_c=9.6e9, df=1.5e6, radius=5e3,
+                                   altitude=2e3, arc_deg=4.0, ntargets=3, patch=20.0,
+                                   snr_db=20.0, taylor=17, upsample=2, seed=1500, name="n1500")
+    sc.phs = scenes.simulate_phs_numpy(sc)
+    n_fft = 1500
+    ip = sar.img_plane_create(sc.nsamples, sc.npulses, sc.delta_r, sc.R_c, with_pixel_locs=False)
+    nu = nv = 48
+    grid = sar.image_grid(nu, nv, sc.delta_r, sc.delta_r, ip["u_hat"], ip["v_hat"])
+    print(f"backprojection N={n_fft} np={sc.npulses} {nu}x{nv}", flush=True)
+    img = run_abi(sar, sc.phs, sc.k_r, sc.taylor, n_fft, sc.delta_r, nu, nv, sc.pos,
+                  grid["pixel_locs"])
+    ref = orc.backprojection(sc.phs, sc.k_r, sc.taylor, n_fft, sc.delta_r, nu, nv, sc.pos,
+                             grid["pixel_locs"], nthreads=NTHREADS)
+    e = rel_l2(img, ref)
+    print(f"   rel_l2 {e:.3e}")
+    assert np.all(np.isfinite(img)) and e < TOL
+    out = np.zeros((nv, nu), np.complex128)
+    rv = sar.backprojection(sc.phs, sc.k_r, sc.taylor, 8193, sc.delta_r, np.zeros(nu),
+                            np.zeros(nv), sc.pos, grid["pixel_locs"], out)
+    assert rv == sar.ERR["param_too_large"]
+
+
 def test_backprojection_n_fft_32768_vs_oracle(sar, orc, scenes, torch_dev):
     """backprojection() at N_fft 32768 (8192 samples, upsample 2) through the Halide-buffer entry
     point, with 2050 pulses: more than one batch of the four-step FFT's scratch (2048 profiles of

@@ -172,8 +172,11 @@ def _tiny(scenes, np_, seed=5, ns=24, taylor=17):
 
 @pytest.mark.parametrize("np_", [15, 16])
 @pytest.mark.parametrize("quirk", [True, False])
-@pytest.mark.parametrize("n_fft", [64, 16])
+@pytest.mark.parametrize("n_fft", [64, 16, 60, 45, 21])
 def test_range_profiles_vs_twin(orc, scenes, np_, quirk, n_fft):
+    """Range profiles of the oracle against numpy (np.fft.fft takes any length; its fftshift rolls
+    by n // 2, which is the reference's (x + ceil(n / 2)) mod n): powers of two, an even and an odd
+    other length, and an odd length shorter than the 24 samples (negative pad)."""
     sc = _tiny(scenes, np_)
     ws = orc.taylor(sc.nsamples, sc.taylor)
     wp = orc.taylor(sc.npulses, sc.taylor)

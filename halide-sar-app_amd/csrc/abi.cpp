@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -136,24 +137,95 @@ int get_context(Context **out) {
 }
 
 // memcpy on up to 8 host threads for large blocks (pageable <-> page-locked staging is bound by
-// one core's copy rate otherwise: ~28 GB/s on one thread, ~100 GB/s on eight).
+// one core's copy rate otherwise: ~28 GB/s on one thread, ~100 GB/s on eight). The helper threads
+// live for the process (a C2 call makes ~35 such copies; a fresh set of threads per copy cost
+// thread start-up each time and left the call at the host scheduler's mercy). Each copy waits only
+// for its own pieces, so callers on several host threads may share the pool.
+class CopyPool {
+  public:
+    static constexpr size_t kMaxThreads = 8;
+
+    static CopyPool &get() {
+        static CopyPool pool;
+        return pool;
+    }
+
+    // Copies n bytes as `parts` pieces, one on the calling thread and the rest on the helpers.
+    void copy(void *dst, const void *src, size_t n, size_t parts) {
+        const size_t chunk = (n / parts + 63) & ~(size_t)63;
+        Batch batch;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            start_workers(parts - 1);
+            for (size_t i = 1; i < parts; ++i) {
+                const size_t lo = i * chunk;
+                if (lo >= n) break;
+                tasks_.push_back({(char *)dst + lo, (const char *)src + lo,
+                                  std::min(chunk, n - lo), &batch});
+                ++batch.left;
+            }
+        }
+        work_cv_.notify_all();
+        std::memcpy(dst, src, std::min(chunk, n));
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return batch.left == 0; });
+    }
+
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        work_cv_.notify_all();
+        for (auto &t : workers_) t.join();
+    }
+
+  private:
+    struct Batch {
+        size_t left = 0;  // pieces not yet copied (guarded by mu_)
+    };
+    struct Task {
+        char *dst;
+        const char *src;
+        size_t n;
+        Batch *batch;
+    };
+
+    void start_workers(size_t want) {  // mu_ held
+        while (workers_.size() < std::min(want, kMaxThreads - 1))
+            workers_.emplace_back([this] { run(); });
+    }
+
+    void run() {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            work_cv_.wait(lk, [&] { return stop_ || !tasks_.empty(); });
+            if (tasks_.empty()) return;  // stop_ with nothing left
+            const Task t = tasks_.back();
+            tasks_.pop_back();
+            lk.unlock();
+            std::memcpy(t.dst, t.src, t.n);
+            lk.lock();
+            if (--t.batch->left == 0) done_cv_.notify_all();
+        }
+    }
+
+    std::mutex mu_;
+    std::condition_variable work_cv_, done_cv_;
+    std::vector<Task> tasks_;
+    std::vector<std::thread> workers_;
+    bool stop_ = false;
+};
+
 void par_copy(void *dst, const void *src, size_t n) {
     constexpr size_t kMin = (size_t)2 << 20;
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const size_t nt = std::min<size_t>({8, hw, n / kMin});
+    const size_t nt = std::min<size_t>({CopyPool::kMaxThreads, hw, n / kMin});
     if (nt <= 1) {
         std::memcpy(dst, src, n);
         return;
     }
-    const size_t chunk = (n / nt + 63) & ~(size_t)63;
-    std::vector<std::thread> th;
-    for (size_t i = 1; i < nt; ++i) {
-        const size_t lo = i * chunk;
-        if (lo >= n) break;
-        th.emplace_back([=] { std::memcpy((char *)dst + lo, (const char *)src + lo, std::min(chunk, n - lo)); });
-    }
-    std::memcpy(dst, src, std::min(chunk, n));
-    for (auto &t : th) t.join();
+    CopyPool::get().copy(dst, src, n, nt);
 }
 
 int hip_fail(hipError_t e, int code, const char *what) {

@@ -329,7 +329,15 @@ def replay(server: Server, factory: JobFactory, rate_hz: float | None, keep_outp
     return recs, wall, kept
 
 
-def summarize(recs, wall) -> dict:
+SMALL_JOB_PIXEL_PULSES = 2 ** 31  # the engine's default fusion bound: jobs at most this size are "small"
+
+
+def summarize(recs, wall, alone_ms=None) -> dict:
+    """Latency, queueing, device and throughput figures of one replay. Latency percentiles are
+    also given per size class (small: at most SMALL_JOB_PIXEL_PULSES pixel-pulses), and, with
+    alone_ms (each job's submission-to-output time when it had the device to itself), the
+    slowdown latency / alone of every job: the tail of a mixed stream is otherwise dominated by
+    its largest jobs' own duration."""
     lat = np.array([r["latency_ms"] for r in recs])
     span = (max(r["t_end_ms"] for r in recs) - min(r["t_submit_ms"] for r in recs)) / 1e3
     pp = sum(r["pp"] for r in recs)
@@ -344,7 +352,26 @@ def summarize(recs, wall) -> dict:
                 host_enqueue_mean_ms=float(hq.mean()), host_enqueue_max_ms=float(hq.max()),
                 host_enqueue_total_s=float(hq.sum() / 1e3),
                 jobs_per_s=len(recs) / span, gpixel_pulses_per_s=pp / span / 1e9,
-                fused_jobs=int(sum(r["fused"] for r in recs)), span_s=span, wall_s=wall)
+                fused_jobs=int(sum(r["fused"] for r in recs)), span_s=span, wall_s=wall,
+                **_by_class(recs, lat, alone_ms))
+
+
+def _by_class(recs, lat, alone_ms) -> dict:
+    out = {}
+    small = np.array([r["pp"] <= SMALL_JOB_PIXEL_PULSES for r in recs])
+    for name, sel in (("small", small), ("large", ~small)):
+        if sel.any():
+            out[f"{name}_jobs"] = int(sel.sum())
+            out[f"{name}_p50_ms"] = float(np.percentile(lat[sel], 50))
+            out[f"{name}_p99_ms"] = float(np.percentile(lat[sel], 99))
+    if alone_ms is not None:
+        slow = lat / np.maximum(np.asarray(alone_ms, np.float64), 1e-3)
+        out["slowdown_p50"] = float(np.percentile(slow, 50))
+        out["slowdown_p99"] = float(np.percentile(slow, 99))
+        for name, sel in (("small", small), ("large", ~small)):
+            if sel.any():
+                out[f"{name}_slowdown_p99"] = float(np.percentile(slow[sel], 99))
+    return out
 
 
 def one_job_per_launch_config(device=0) -> dict:
@@ -366,16 +393,22 @@ def benchmark(specs, load=0.7, seed_base=4000, device=0, log=print, **server_kw)
     with Server(device=device, **server_kw) as srv, \
             Server(**one_job_per_launch_config(device)) as base:
         caps = {}
-        for name, s in (("serving", srv), ("one_job_per_launch", base)):
+        alone = None
+        for name, s in (("one_job_per_launch", base), ("serving", srv)):
             replay(s, factory, None)
-            caps[name] = summarize(*replay(s, factory, None)[:2])
+            recs, wall, _ = replay(s, factory, None)
+            if alone is None:
+                # one job at a time: each job's own submission-to-output time on an idle device
+                # (its queueing behind earlier jobs excluded), the reference for slowdowns
+                alone = [r["latency_ms"] - r["queue_ms"] for r in recs]
+            caps[name] = summarize(recs, wall, alone)
             log(f"{name}: capacity {caps[name]['jobs_per_s']:.1f} jobs/s")
         rate = load * caps["serving"]["jobs_per_s"]
         rate_low = load * caps["one_job_per_launch"]["jobs_per_s"]
         for name, s in (("serving", srv), ("one_job_per_launch", base)):
             res[name] = dict(capacity=caps[name])
             for key, r in (("stream", rate), ("stream_low", rate_low)):
-                res[name][key] = summarize(*replay(s, factory, r)[:2])
+                res[name][key] = summarize(*replay(s, factory, r)[:2], alone)
                 log(f"{name}: Poisson {r:.1f}/s: {res[name][key]}")
     res["arrival_rate_hz"] = rate
     res["arrival_rate_low_hz"] = rate_low

@@ -431,19 +431,13 @@ __device__ inline void store_profile_row(const float2 *buf, float *red, int N,
     __syncthreads();
 }
 
-// One block of 512 threads per range profile p, the FFT in place in one LDS row. NB4: radix-4
-// butterflies per thread and pass, N / (4 * kFftThreads) rounded up (4 up to N = 8192, 8 at
-// 16384), so that the registers of smaller rows are sized for them.
+// Range profile p in one block of 512 threads, the FFT in place in one LDS row.
 template <typename QT, int NB4>
-__global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
-    const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
-    const double *__restrict__ wp, int ns, int np, int N, int logN,
+__device__ inline void range_profile_one(
+    float2 *buf, float *red, int p, const float2 *__restrict__ phs, const float *__restrict__ k_r,
+    const double *__restrict__ ws, const double *__restrict__ wp, int ns, int np, int N, int logN,
     const float2 *__restrict__ tw, const float2 *__restrict__ rot, int ext, int len,
-    QT *__restrict__ q, float *__restrict__ qscale) {
-    extern __shared__ float2 buf[];
-    __shared__ float red[kFftThreads / 64];
-    const int p = blockIdx.x;
-    const int tid = threadIdx.x;
+    QT *__restrict__ q, float *__restrict__ qscale, int tid) {
     load_profile_row<kFftThreads>(buf, phs, k_r, ws, wp, ns, N, profile_source(p, np), tid);
     __syncthreads();
     int Ns = 1;
@@ -454,6 +448,37 @@ __global__ __launch_bounds__(kFftThreads) void range_profile_kernel(
     for (; Ns < N; Ns *= 4)
         fft_pass<4, NB4>(buf, N, Ns, fft_stage_twiddles(tw, N, logN, Ns), tid);
     store_profile_row<QT, kFftThreads>(buf, red, N, rot, ext, len, q, qscale, p, tid);
+}
+
+// Range profiles, one block of 512 threads per profile. NB4: radix-4 butterflies per thread and
+// pass, N / (4 * kFftThreads) rounded up (4 up to N = 8192, 8 at 16384), so that the registers of
+// smaller rows are sized for them.
+//   NB4 = 4: two blocks fit a CU's LDS (4 waves per SIMD, at most 128 VGPRs); one profile per
+//            block (blockIdx.x), the grid holds every profile.
+//   NB4 = 8: one block per CU (128 KiB of LDS); blocks loop over profiles p = blockIdx.x +
+//            k gridDim.x on a grid of kRowBlocksPerCu blocks per CU, so that a block moves on to
+//            its next profile while the previous one's row stores drain (C3: 1.29 -> 1.23 ms;
+//            at N = 8192 the loop made the kernel slower: 0.118 -> 0.123 ms).
+constexpr int kRowBlocksPerCu = 4;
+
+template <typename QT, int NB4>
+__global__ __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(NB4 == 4 ? 4 : 2)))
+void range_profile_kernel(
+    const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
+    const double *__restrict__ wp, int ns, int np, int N, int logN,
+    const float2 *__restrict__ tw, const float2 *__restrict__ rot, int ext, int len,
+    QT *__restrict__ q, float *__restrict__ qscale) {
+    extern __shared__ float2 buf[];
+    __shared__ float red[kFftThreads / 64];
+    const int tid = threadIdx.x;
+    if constexpr (NB4 == 4) {
+        range_profile_one<QT, NB4>(buf, red, blockIdx.x, phs, k_r, ws, wp, ns, np, N, logN, tw,
+                                   rot, ext, len, q, qscale, tid);
+    } else {
+        for (int p = blockIdx.x; p < np; p += gridDim.x)
+            range_profile_one<QT, NB4>(buf, red, p, phs, k_r, ws, wp, ns, np, N, logN, tw, rot,
+                                       ext, len, q, qscale, tid);
+    }
 }
 
 // Range profiles of an N_fft that is not a power of two (N <= kMaxFftBluestein), by Bluestein's
@@ -2246,6 +2271,13 @@ hipError_t launch_range_profiles(const void *phs, const float *k_r, const double
     const size_t lds = sizeof(float2) * (size_t)n_fft;
     static_assert(kMaxFft == 8 * 4 * kFftThreads, "NB4 = 8 covers the largest one-row FFT");
     const bool big = n_fft > 4 * 4 * kFftThreads;  // NB4 = 8, else 4
+    unsigned rp_grid = (unsigned)np;  // NB4 = 4: one block per profile
+    if (big) {
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        rp_grid = (unsigned)std::min<int64_t>(np, (int64_t)kRowBlocksPerCu * std::max(cus, 1));
+    }
     const void *fn = q_format == SARBP_Q_C16
                          ? (big ? (const void *)&range_profile_kernel<QPairF16, 8>
                                 : (const void *)&range_profile_kernel<QPairF16, 4>)
@@ -2261,21 +2293,21 @@ hipError_t launch_range_profiles(const void *phs, const float *k_r, const double
     if (q_format == SARBP_Q_C16) {
         QPairF16 *qq = (QPairF16 *)q;
         if (big)
-            hipLaunchKernelGGL((range_profile_kernel<QPairF16, 8>), dim3(np), dim3(kFftThreads), lds,
+            hipLaunchKernelGGL((range_profile_kernel<QPairF16, 8>), dim3(rp_grid), dim3(kFftThreads), lds,
                                stream, (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
                                rot_c, lay.ext, lay.len, qq, qscale);
         else
-            hipLaunchKernelGGL((range_profile_kernel<QPairF16, 4>), dim3(np), dim3(kFftThreads), lds,
+            hipLaunchKernelGGL((range_profile_kernel<QPairF16, 4>), dim3(rp_grid), dim3(kFftThreads), lds,
                                stream, (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
                                rot_c, lay.ext, lay.len, qq, qscale);
     } else {
         QPairF32 *qq = (QPairF32 *)q;
         if (big)
-            hipLaunchKernelGGL((range_profile_kernel<QPairF32, 8>), dim3(np), dim3(kFftThreads), lds,
+            hipLaunchKernelGGL((range_profile_kernel<QPairF32, 8>), dim3(rp_grid), dim3(kFftThreads), lds,
                                stream, (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
                                rot_c, lay.ext, lay.len, qq, (float *)nullptr);
         else
-            hipLaunchKernelGGL((range_profile_kernel<QPairF32, 4>), dim3(np), dim3(kFftThreads), lds,
+            hipLaunchKernelGGL((range_profile_kernel<QPairF32, 4>), dim3(rp_grid), dim3(kFftThreads), lds,
                                stream, (const float2 *)phs, k_r, ws, wp, ns, np, n_fft, logN, tw,
                                rot_c, lay.ext, lay.len, qq, (float *)nullptr);
     }

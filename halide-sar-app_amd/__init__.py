@@ -191,6 +191,13 @@ def last_error() -> str:
     return (_lib.sarbp_last_error() or b"").decode()
 
 
+def abi_last_call_bands() -> tuple[int, int]:
+    """(bands on the image plane, bands) of this thread's last backprojection() call."""
+    g, n = ctypes.c_int32(0), ctypes.c_int32(0)
+    _lib.sarbp_abi_last_call_bands(ctypes.byref(g), ctypes.byref(n))
+    return g.value, n.value
+
+
 def device_count() -> int:
     return int(_lib.sarbp_device_count())
 

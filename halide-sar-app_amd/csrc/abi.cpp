@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <map>
@@ -136,42 +137,41 @@ int get_context(Context **out) {
     return 0;
 }
 
-// memcpy on up to 8 host threads for large blocks (pageable <-> page-locked staging is bound by
-// one core's copy rate otherwise: ~28 GB/s on one thread, ~100 GB/s on eight). The helper threads
-// live for the process (a C2 call makes ~35 such copies; a fresh set of threads per copy cost
-// thread start-up each time and left the call at the host scheduler's mercy). Each copy waits only
-// for its own pieces, so callers on several host threads may share the pool.
-class CopyPool {
+// Host work of a call spread over up to 8 threads: the copies between the caller's buffers and
+// the pinned staging (pageable <-> page-locked copies are bound by one core's copy rate otherwise:
+// ~28 GB/s on one thread, ~100 GB/s on eight) and the check of pixel locations against the image
+// plane. The helper threads live for the process (a C2 call makes ~35 such copies; a fresh set of
+// threads per copy cost thread start-up each time). Each batch waits only for its own pieces, so
+// callers on several host threads may share the pool.
+class HostPool {
   public:
     static constexpr size_t kMaxThreads = 8;
 
-    static CopyPool &get() {
-        static CopyPool pool;
+    static HostPool &get() {
+        static HostPool pool;
         return pool;
     }
 
-    // Copies n bytes as `parts` pieces, one on the calling thread and the rest on the helpers.
-    void copy(void *dst, const void *src, size_t n, size_t parts) {
-        const size_t chunk = (n / parts + 63) & ~(size_t)63;
+    // Runs fn(0) .. fn(parts - 1), part 0 on the calling thread and the rest on the helpers.
+    template <typename F>
+    void run(size_t parts, const F &fn) {
         Batch batch;
+        Thunk thunk{&fn, [](const void *f, size_t i) { (*static_cast<const F *>(f))(i); }};
         {
             std::lock_guard<std::mutex> lk(mu_);
             start_workers(parts - 1);
             for (size_t i = 1; i < parts; ++i) {
-                const size_t lo = i * chunk;
-                if (lo >= n) break;
-                tasks_.push_back({(char *)dst + lo, (const char *)src + lo,
-                                  std::min(chunk, n - lo), &batch});
+                tasks_.push_back({thunk, i, &batch});
                 ++batch.left;
             }
         }
         work_cv_.notify_all();
-        std::memcpy(dst, src, std::min(chunk, n));
+        fn(0);
         std::unique_lock<std::mutex> lk(mu_);
         done_cv_.wait(lk, [&] { return batch.left == 0; });
     }
 
-    ~CopyPool() {
+    ~HostPool() {
         {
             std::lock_guard<std::mutex> lk(mu_);
             stop_ = true;
@@ -182,21 +182,24 @@ class CopyPool {
 
   private:
     struct Batch {
-        size_t left = 0;  // pieces not yet copied (guarded by mu_)
+        size_t left = 0;  // pieces not yet done (guarded by mu_)
+    };
+    struct Thunk {
+        const void *fn;
+        void (*call)(const void *, size_t);
     };
     struct Task {
-        char *dst;
-        const char *src;
-        size_t n;
+        Thunk thunk;
+        size_t index;
         Batch *batch;
     };
 
     void start_workers(size_t want) {  // mu_ held
         while (workers_.size() < std::min(want, kMaxThreads - 1))
-            workers_.emplace_back([this] { run(); });
+            workers_.emplace_back([this] { work(); });
     }
 
-    void run() {
+    void work() {
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
             work_cv_.wait(lk, [&] { return stop_ || !tasks_.empty(); });
@@ -204,7 +207,7 @@ class CopyPool {
             const Task t = tasks_.back();
             tasks_.pop_back();
             lk.unlock();
-            std::memcpy(t.dst, t.src, t.n);
+            t.thunk.call(t.thunk.fn, t.index);
             lk.lock();
             if (--t.batch->left == 0) done_cv_.notify_all();
         }
@@ -217,16 +220,115 @@ class CopyPool {
     bool stop_ = false;
 };
 
-void par_copy(void *dst, const void *src, size_t n) {
+// Pieces for n bytes of host work: one per 2 MB, at most the pool's threads and the host's cores.
+size_t host_parts(size_t n) {
     constexpr size_t kMin = (size_t)2 << 20;
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const size_t nt = std::min<size_t>({CopyPool::kMaxThreads, hw, n / kMin});
-    if (nt <= 1) {
+    return std::max<size_t>(1, std::min<size_t>({HostPool::kMaxThreads, hw, n / kMin}));
+}
+
+void par_copy(void *dst, const void *src, size_t n) {
+    const size_t parts = host_parts(n);
+    if (parts <= 1) {
         std::memcpy(dst, src, n);
         return;
     }
-    CopyPool::get().copy(dst, src, n, nt);
+    const size_t chunk = (n / parts + 63) & ~(size_t)63;
+    HostPool::get().run(parts, [&](size_t i) {
+        const size_t lo = i * chunk;
+        if (lo < n) std::memcpy((char *)dst + lo, (const char *)src + lo, std::min(chunk, n - lo));
+    });
 }
+
+// The image plane of img_plane_create (img_plane.cpp:12-14,77-113), which pixel_locs usually
+// holds: u[x] = (floor(-nu/2) + x) d_u, v[y] likewise, and for grid row y and column x (pixel
+// x + nu y) r = (0 + u_hat u[x]) + v_hat v[y], rounded in that order. The reference reads only
+// the extents of u and v (backprojection.cpp:71-72), but callers pass the plane's own arrays. When
+// they are that linspace, u_hat and v_hat follow from the two pixels where v or u is 0, whose
+// locations are single products. A band of the call whose every pixel location lies within tol of
+// the plane then runs in grid mode: the accumulation generates the locations (sarbp_plan_set_grid)
+// and takes the cubic form of the range expansion, and the band's pixel locations are neither
+// copied nor uploaded. Any other band reads r as before. tol = 2^-40 of the plane's extent (3e-10 m
+// for a 300 m scene: a range error of ~1e-7 rad of phase at X band).
+struct PlaneGuess {
+    bool on = false;
+    double d_u = 0.0, d_v = 0.0, uh[3] = {}, vh[3] = {}, tol = 0.0;
+    std::vector<double> u, v;
+    const double *rc[3] = {};  // pixel i's component c at rc[c][i - rmin]
+    int64_t rmin = 0;
+    int nu = 0;
+};
+
+bool linspace_of(const halide_buffer_t *b, int n, std::vector<double> &out, double &step) {
+    if (n < 2 || !b->host) return false;
+    out.resize((size_t)n);
+    const double *h = (const double *)b->host;
+    for (int x = 0; x < n; ++x) {
+        const int64_t co[1] = {b->dim[0].min + x};
+        out[x] = h[offset_of(b, co)];
+    }
+    const int k0 = -((n + 1) / 2);                   // floor(-n / 2) (img_plane.cpp:12-14)
+    const int xa = -k0 + 1 < n ? -k0 + 1 : -k0 - 1;  // the sample at +-1 step
+    step = out[xa] * (double)(k0 + xa);              // k0 + xa = +-1
+    if (!(std::fabs(step) > 0.0) || !std::isfinite(step)) return false;
+    for (int x = 0; x < n; ++x)
+        if (out[x] != (double)(k0 + x) * step) return false;
+    return true;
+}
+
+void guess_plane(const halide_buffer_t *u, const halide_buffer_t *v, const halide_buffer_t *r,
+                 int nu, int nv, int64_t rmin, int64_t rext, PlaneGuess &g) {
+    g.on = false;
+    if (!r->host || r->dim[0].stride != 1) return;
+    for (int c = 0; c < 3; ++c)
+        if (!in_dim(r, 1, c)) return;
+    if (!linspace_of(u, nu, g.u, g.d_u) || !linspace_of(v, nv, g.v, g.d_v)) return;
+    const int x0 = (nu + 1) / 2, y0 = (nv + 1) / 2;  // u[x0] = 0, v[y0] = 0
+    const int xa = x0 + 1 < nu ? x0 + 1 : x0 - 1, ya = y0 + 1 < nv ? y0 + 1 : y0 - 1;
+    const int64_t ia = (int64_t)y0 * nu + xa, ib = (int64_t)ya * nu + x0;
+    if (ia < rmin || ia >= rmin + rext || ib < rmin || ib >= rmin + rext) return;
+    double umax = 0.0, vmax = 0.0;
+    for (double a : g.u) umax = std::max(umax, std::fabs(a));
+    for (double a : g.v) vmax = std::max(vmax, std::fabs(a));
+    const double *rs = (const double *)r->host;
+    for (int c = 0; c < 3; ++c) {
+        const int64_t co[2] = {rmin, c};
+        g.rc[c] = rs + offset_of(r, co);
+        g.uh[c] = g.rc[c][ia - rmin] / g.u[xa];  // r(xa, y0) = u_hat u[xa]
+        g.vh[c] = g.rc[c][ib - rmin] / g.v[ya];  // r(x0, ya) = v_hat v[ya]
+        if (!std::isfinite(g.uh[c]) || !std::isfinite(g.vh[c])) return;
+    }
+    g.tol = std::ldexp(umax + vmax, -40);
+    g.rmin = rmin;
+    g.nu = nu;
+    g.on = true;
+}
+
+// Whether pixels [lo, hi) of r lie within g.tol of the plane (checked on the host pool).
+bool band_on_plane(const PlaneGuess &g, int64_t lo, int64_t hi) {
+    if (hi <= lo) return true;
+    const size_t parts = host_parts((size_t)(hi - lo) * 24);
+    const int64_t chunk = (hi - lo + (int64_t)parts - 1) / (int64_t)parts;
+    std::vector<char> ok(parts, 1);
+    HostPool::get().run(parts, [&](size_t k) {
+        const int64_t a = lo + (int64_t)k * chunk, b = std::min(hi, a + chunk);
+        for (int64_t i = a; i < b; ++i) {
+            const int64_t y = i / g.nu, x = i - y * g.nu;
+            for (int c = 0; c < 3; ++c) {
+                const double m = (0.0 + g.uh[c] * g.u[x]) + g.vh[c] * g.v[y];
+                if (!(std::fabs(g.rc[c][i - g.rmin] - m) <= g.tol)) {
+                    ok[k] = 0;
+                    return;
+                }
+            }
+        }
+    });
+    for (char o : ok)
+        if (!o) return false;
+    return true;
+}
+
+thread_local int g_last_grid_bands = 0, g_last_bands = 0;
 
 int hip_fail(hipError_t e, int code, const char *what) {
     set_error("%s: %s", what, hipGetErrorString(e));
@@ -418,6 +520,11 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
     if ((e = hipMemcpyAsync(dbase + phs_sent, h + phs_sent, b_phs + b_kr + b_pos - phs_sent,
                             hipMemcpyHostToDevice, s1)) != hipSuccess)
         return drain(halide_error_code_copy_to_device_failed, "input upload");
+    PlaneGuess plane;
+    guess_plane(u, v, r, nu, nv, rmin, rext, plane);
+    if (plane.on && sarbp_plan_set_grid(plan, plane.d_u, plane.d_v, plane.uh, plane.vh) != 0)
+        plane.on = false;
+    if (!plane.on) (void)sarbp_plan_set_grid(plan, 0.0, 0.0, nullptr, nullptr);
     rv = plan_prepare(plan, dbase, dbase + b_phs, dbase + b_phs + b_kr, s1);
     if (rv) {
         (void)hipStreamSynchronize(s1);
@@ -447,11 +554,32 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
     // r = constant_exterior(r_in, 0.0) (backprojection.cpp:77): a band reads r only at its own pixels,
     // px = nu * (nv - y - 1) + x (backprojection.cpp:130), a contiguous index range per band.
     const double *rs = (const double *)r->host;
+    g_last_grid_bands = 0;
+    g_last_bands = nbands;
     for (int b = 0; b < nbands; ++b) {
         const int row0 = b * band, nrows = std::min(band, oh - row0);
         const int64_t ylo = oy0 + row0, yhi = ylo + nrows - 1;
-        const int64_t lo = std::max<int64_t>(rmin, (int64_t)nu * (nv - yhi - 1) + ox0);
-        const int64_t hi = std::min<int64_t>(rmin + rext, (int64_t)nu * (nv - ylo - 1) + ox0 + ow);
+        const int64_t plo = (int64_t)nu * (nv - yhi - 1) + ox0;
+        const int64_t phi = (int64_t)nu * (nv - ylo - 1) + ox0 + ow;
+        const int64_t lo = std::max<int64_t>(rmin, plo), hi = std::min<int64_t>(rmin + rext, phi);
+        hipStream_t sc = (b & 1) ? s3 : s1;
+        // a band on the image plane: no pixel locations to move. A band with pixels outside r's
+        // extent reads r (0 there); once one band is off the plane, the rest read r unchecked.
+        if (plane.on &&
+            (plane.on = plo >= rmin && phi <= rmin + rext && band_on_plane(plane, lo, hi))) {
+            ++g_last_grid_bands;
+            rv = plan_accumulate_rows(plan, dr, rmin, rext, row0, nrows,
+                                      (double *)ctx->d_out.p + (size_t)2 * ow * row0, sc, true);
+            if (rv) {
+                (void)hipStreamSynchronize(s1);
+                (void)hipStreamSynchronize(s2);
+                (void)hipStreamSynchronize(s3);
+                return rv;
+            }
+            if ((e = hipEventRecord(ev_acc[b], sc)) != hipSuccess)
+                return drain(halide_error_code_device_run_failed, "band ordering");
+            continue;
+        }
         if (hi > lo) {
             const int64_t n = hi - lo;
             for (int64_t c = 0; c < 3; ++c) {
@@ -472,12 +600,11 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
                 if (e != hipSuccess) return drain(halide_error_code_copy_to_device_failed, "r upload");
             }
         }
-        hipStream_t sc = (b & 1) ? s3 : s1;
         if ((e = hipEventRecord(ev_r[b], s2)) != hipSuccess ||
             (e = hipStreamWaitEvent(sc, ev_r[b], 0)) != hipSuccess)
             return drain(halide_error_code_device_run_failed, "band ordering");
         rv = plan_accumulate_rows(plan, dr, rmin, rext, row0, nrows,
-                                  (double *)ctx->d_out.p + (size_t)2 * ow * row0, sc);
+                                  (double *)ctx->d_out.p + (size_t)2 * ow * row0, sc, false);
         if (rv) {
             (void)hipStreamSynchronize(s1);
             (void)hipStreamSynchronize(s2);
@@ -716,6 +843,12 @@ void abi_dft_init(size_t N_fft) {
 // kept until it exits, so that calls in flight on other threads are unaffected.
 void abi_dft_destroy() {}
 }  // namespace sarbp
+
+extern "C" int sarbp_abi_last_call_bands(int32_t *grid_bands, int32_t *bands) {
+    if (grid_bands) *grid_bands = g_last_grid_bands;
+    if (bands) *bands = g_last_bands;
+    return 0;
+}
 
 extern "C" void dft_init_fftw(size_t N_fft) { abi_dft_init(N_fft); }
 

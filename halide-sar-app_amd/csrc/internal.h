@@ -193,8 +193,11 @@ constexpr int kMaxFftBluestein = kMaxFft / 2;
 // for bit.
 int plan_prepare(sarbp_plan *p, const void *phs_dev, const void *k_r_dev, const void *pos_dev,
                  hipStream_t stream);
+// use_grid = false runs the rows from r even when the plan has a grid (sarbp_plan_set_grid): the
+// pulse table holds the terms of both forms, so the bands of one call may mix them.
 int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int64_t r_extent,
-                         int row0, int nrows, void *out_rows, hipStream_t stream);
+                         int row0, int nrows, void *out_rows, hipStream_t stream,
+                         bool use_grid = true);
 int plan_tile_rows(const sarbp_plan *p);
 // Tiles of the plan's accumulation kernel resident on its device at once (0 if unknown).
 int plan_resident_tiles(const sarbp_plan *p);

@@ -475,7 +475,7 @@ int plan_prepare(sarbp_plan *p, const void *phs_dev, const void *k_r_dev, const 
 }
 
 int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int64_t r_extent,
-                         int row0, int nrows, void *out_rows, hipStream_t stream) {
+                         int row0, int nrows, void *out_rows, hipStream_t stream, bool use_grid) {
     DeviceGuard g(p->d.device);
     const sarbp_plan_desc &d = p->d;
     BpArgs a{};
@@ -503,7 +503,7 @@ int plan_accumulate_rows(sarbp_plan *p, const double *r_dev, int64_t r_min, int6
     a.full_h = d.height;
     a.cubic = 0;
     a.out = (double2 *)out_rows;
-    a.g = p->grid;
+    a.g = use_grid ? p->grid : PixelGrid{};
     a.absmax = p->absmax;
     a.paths = p->paths;
     a.range_df = d.range_mode == SARBP_RANGE_DF32;

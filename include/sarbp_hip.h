@@ -109,6 +109,13 @@ int sarbp_get_default_policies(int *norm_r0_policy, int *taylor_den_policy);
 /* Last error message of the calling thread ("" if none). */
 const char *sarbp_last_error(void);
 
+/* How the calling thread's last backprojection() / backprojection_cuda() call formed its image:
+ * bands in all, and bands whose pixel locations were found on the image plane of the call's u, v
+ * buffers (so that the accumulation generated them, in grid mode, instead of reading r). No
+ * counterpart in the reference; for tests and for callers checking that their pixel_locs take the
+ * faster path. */
+int sarbp_abi_last_call_bands(int32_t *grid_bands, int32_t *bands);
+
 /* Number of visible GPUs (0 when none; never aborts). */
 int sarbp_device_count(void);
 

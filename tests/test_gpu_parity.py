@@ -235,6 +235,71 @@ def test_backprojection_n_fft_not_power_of_two_vs_oracle(sar, orc, scenes, torch
     assert rv == sar.ERR["param_too_large"]
 
 
+def test_abi_pixel_locs_on_image_plane_take_grid_mode(sar, orc, torch_dev):
+    """backprojection() with the u, v arrays of the image plane (img_plane.cpp:12-14) and pixel_locs
+    on that plane: every band is found on the plane and accumulated in grid mode (no pixel
+    locations uploaded), the image within 1e-4 of the oracle and within 1e-6 of the same call
+    reading r (u, v zero, as only their extents matter to the reference). With one pixel location
+    moved by 1 um, the bands from that pixel's band on read r."""
+    g = np.load(os.path.join(GOLDEN, "c1_64.npz"))
+    args = (g["phs"], g["k_r"], int(g["taylor"]), int(g["n_fft"]), float(g["delta_r"]))
+    # the img_plane_create plane itself (512 x 512, one band)
+    ip = sar.img_plane_create(424, 469, float(g["delta_r"]), g["R_c"])
+    out = np.zeros((ip["nv"], ip["nu"]), np.complex128)
+    print(f"plane: img_plane_create {ip['nu']}x{ip['nv']}", flush=True)
+    assert sar.backprojection(*args, ip["u"], ip["v"], g["pos"], ip["pixel_locs"], out) == 0
+    gb, nb = sar.abi_last_call_bands()
+    print(f"   bands on the plane {gb} of {nb}", flush=True)
+    assert nb >= 1 and gb == nb
+    img_r = run_abi(sar, *args, ip["nu"], ip["nv"], g["pos"], ip["pixel_locs"])
+    assert sar.abi_last_call_bands()[0] == 0
+    assert rel_l2(out, img_r) < 1e-6
+    # a 2048 x 2048 grid of the same plane at a quarter of the pixel spacing: several bands
+    nu = nv = 2048
+    grid = sar.image_grid(nu, nv, ip["d_u"] / 4, ip["d_v"] / 4, ip["u_hat"], ip["v_hat"])
+    print(f"plane: {nu}x{nv} grid", flush=True)
+    out = np.zeros((nv, nu), np.complex128)
+    assert sar.backprojection(*args, grid["u"], grid["v"], g["pos"], grid["pixel_locs"], out) == 0
+    gb, nb = sar.abi_last_call_bands()
+    print(f"   bands on the plane {gb} of {nb}", flush=True)
+    assert nb >= 2 and gb == nb
+    img_r = run_abi(sar, *args, nu, nv, g["pos"], grid["pixel_locs"])
+    assert sar.abi_last_call_bands()[0] == 0
+    ref = orc.backprojection(*args, nu, nv, g["pos"], grid["pixel_locs"], nthreads=NTHREADS)
+    e, e_r = rel_l2(out, ref), rel_l2(out, img_r)
+    print(f"   rel_l2 vs oracle {e:.3e}, vs the r-mode call {e_r:.3e}", flush=True)
+    assert np.all(np.isfinite(out)) and e < TOL and e_r < 1e-6
+    moved = grid["pixel_locs"].copy()
+    k = (nv // 4) * nu + nu // 3  # grid row nv / 4: output row 3 nv / 4 - 1, past the first band
+    moved[0, k] += 1e-6
+    out2 = np.zeros((nv, nu), np.complex128)
+    assert sar.backprojection(*args, grid["u"], grid["v"], g["pos"], moved, out2) == 0
+    gb2, nb2 = sar.abi_last_call_bands()
+    print(f"   one pixel moved: bands on the plane {gb2} of {nb2}", flush=True)
+    assert 0 < gb2 < nb2
+    e2 = rel_l2(out2, ref)
+    assert np.all(np.isfinite(out2)) and e2 < TOL
+    # pixels away from the moved one agree with the all-plane image to the forms' rounding
+    mask = np.ones((nv, nu), bool)
+    mask[nv - 1 - nv // 4, nu // 3] = False
+    assert rel_l2(out2[mask], out[mask]) < 1e-6
+    # r from pixel 5 nu on (Halide min 5 nu): output rows nv - 5.. read r = 0 (constant_exterior,
+    # backprojection.cpp:77), so the last band reads r while the others stay on the plane
+    pl = grid["pixel_locs"]
+    rb = sar.Buffer(pl[:, 5 * nu:], halide_dims=[(5 * nu, nu * nv - 5 * nu, 1), (0, 3, nu * nv)])
+    out3 = np.zeros((nv, nu), np.complex128)
+    assert sar.backprojection(*args, grid["u"], grid["v"], g["pos"], rb, out3) == 0
+    gb3, nb3 = sar.abi_last_call_bands()
+    print(f"   r from pixel 5 nu: bands on the plane {gb3} of {nb3}", flush=True)
+    assert gb3 == nb3 - 1
+    padded = pl.copy()
+    padded[:, :5 * nu] = 0.0
+    ref3 = orc.backprojection(*args, nu, nv, g["pos"], padded, nthreads=NTHREADS)
+    e3, e3_rows = rel_l2(out3, ref3), rel_l2(out3[nv - 5:], ref3[nv - 5:])
+    print(f"   rel_l2 vs oracle {e3:.3e}, rows read at r = 0 {e3_rows:.3e}", flush=True)
+    assert np.all(np.isfinite(out3)) and e3 < TOL and e3_rows < TOL
+
+
 def test_backprojection_n_fft_32768_vs_oracle(sar, orc, scenes, torch_dev):
     """backprojection() at N_fft 32768 (8192 samples, upsample 2) through the Halide-buffer entry
     point, with 2050 pulses: more than one batch of the four-step FFT's scratch (2048 profiles of

@@ -208,8 +208,9 @@ def main():
         abi_ms = sorted(times)[2]
         plane_bands = sar.abi_last_call_bands()
         abi = {"what": ("one backprojection() call with host buffers, as sarbp.cpp:276-280 times "
-                        "it: PCIe copies of phs, pixel_locs (24 B/pixel) and the f64 image "
-                        "included; median of 5"),
+                        "it: PCIe copies of phs and the f64 image included; pixel_locs "
+                        "(24 B/pixel) checked on the host against the image plane of u, v and "
+                        "uploaded only for bands off it; median of 5"),
                "ms": round(abi_ms, 3), "gpixel_pulses_per_s": round(pp / abi_ms / 1e6, 3),
                "runs_ms": [round(t, 3) for t in times],
                "bands_on_image_plane": f"{plane_bands[0]} of {plane_bands[1]}",

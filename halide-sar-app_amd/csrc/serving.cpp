@@ -9,7 +9,9 @@
 //
 // Work flows through streams with one role each, so uploads, kernels and downloads of different
 // jobs overlap:
-//   stage    uploads of every job (copies only)
+//   stage    uploads of every job (copies only): a small job's inputs at admission, a larger
+//            job's phase history and pixel locations in chunks, at most kUploadChunksAhead
+//            outstanding, so that a small job's upload waits for at most that many chunks
 //   prep     pulse table and range profiles of small jobs (nu * nv * npulses at most the fusion
 //            size)
 //   big[i]   pulse table, range profiles and accumulation of larger jobs, in row bands (the least
@@ -59,8 +61,14 @@ constexpr int kFuseSlots = 2;                       // fused launches in flight 
 // holds this many pixels whatever the tile height: 4096 tiles of 16 x 32 (two device waves at 8
 // tiles per CU) or 2048 of 16 x 64 (two waves at 4 per CU), so that its duration does not grow with
 // the tile the tuner picks.
+// While small jobs are in flight, bands are shorter (kBandPixelPulsesShared, but at least one
+// device wave of tiles) and only one is enqueued ahead, so that a small job's kernels wait for at
+// most about one short band.
 constexpr int64_t kBandPixels = 4096 * 16 * 32;
+constexpr int64_t kBandPixelPulsesShared = 4'000'000'000LL;
 constexpr int kBandsAhead = 2;
+constexpr size_t kUploadChunkBytes = (size_t)8 << 20;  // ~0.15 ms over PCIe 5 x16
+constexpr int kUploadChunksAhead = 2;
 // Pixel-pulses per fused launch at most (~6 ms of accumulation on an idle device), so that a fused
 // launch holds the hardware queue it shares with other streams no longer than a band does.
 constexpr int64_t kFusePixelPulses = 8'000'000'000LL;
@@ -177,6 +185,9 @@ struct Job {
     void *stage = nullptr;       // page-locked copy of k_r, pos, u, v, ws, wp
     size_t stage_cls = 0;
     int64_t t_submit = 0, t_admit = 0, t_end = 0;
+    // when the scheduler saw each step end or began it (0: not reached): inputs uploaded, range
+    // profiles computed, accumulation enqueued (fused launch or first band), accumulation complete
+    int64_t t_up = 0, t_ready = 0, t_launch = 0, t_acc = 0;
     int64_t pp = 0;  // pixel-pulses of the job
     int stream = 0;
     int ppt = 0;
@@ -190,6 +201,11 @@ struct Job {
     size_t off = SIZE_MAX;       // arena block
     uint8_t *dev = nullptr;
     hipEvent_t ev_up = nullptr;     // inputs in device memory
+    bool up_recorded = false;       // ev_up is recorded (after a chunked upload's last chunk)
+    size_t up_sent = 0, up_total = 0;            // larger jobs: bytes of phs and r enqueued, of all
+    size_t dl_sent = 0;                          // larger jobs: bytes of the output enqueued
+    hipEvent_t ev_chunk[kUploadChunksAhead] = {};  // larger jobs: the outstanding copy chunks
+    int chunks_enqueued = 0, chunks_done = 0;
     hipEvent_t ev_ready = nullptr;  // small: prepared for the fused launch
     hipEvent_t ev_acc = nullptr;    // large: accumulated
     hipEvent_t ev_band[2] = {};     // large: completion of the outstanding accumulation bands
@@ -308,7 +324,13 @@ int check_job(const sarbp_job *j) {
     return 0;
 }
 
-// Enqueue the uploads of an admitted job on the stage stream, then its ev_up.
+bool event_done(hipEvent_t ev) { return hipEventQuery(ev) == hipSuccess; }
+int advance_upload(sarbp_server *s, Job &j);
+int advance_download(sarbp_server *s, Job &j);
+
+// Enqueue the uploads of an admitted job on the stage stream: a small job's at once, then its
+// ev_up; a larger job's staged arrays at once and its phase history and pixel locations by
+// advance_upload.
 int upload_job(sarbp_server *s, Job &j) {
     hipStream_t st = s->stage;
     const sarbp_job &d = j.d;
@@ -330,13 +352,51 @@ int upload_job(sarbp_server *s, Job &j) {
     std::memcpy(h + j.ar.pos, d.pos, 12 * np);
     std::memcpy(h + j.ar.ws, j.ws.data(), 8 * ns);
     std::memcpy(h + j.ar.wp, j.wp.data(), 8 * np);
-    hipError_t e = hipMemcpyAsync(b + j.ar.phs, d.phs, 8 * np * ns, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b + j.ar.host_lo, j.stage, j.ar.host_hi - j.ar.host_lo,
-                           hipMemcpyHostToDevice, st);
+    hipError_t e = hipMemcpyAsync(b + j.ar.host_lo, j.stage, j.ar.host_hi - j.ar.host_lo,
+                                  hipMemcpyHostToDevice, st);
+    j.up_total = 8 * np * ns + (d.pixel_locs ? 24 * npix : 0);
+    if (e == hipSuccess && !j.small && j.up_total > kUploadChunkBytes * kUploadChunksAhead)
+        return advance_upload(s, j);
+    if (e == hipSuccess) e = hipMemcpyAsync(b + j.ar.phs, d.phs, 8 * np * ns, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && d.pixel_locs)
         e = hipMemcpyAsync(b + j.ar.r, d.pixel_locs, 24 * npix, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipEventRecord(j.ev_up, st);
+    if (e != hipSuccess) {
+        set_error("job %llu: upload failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
+        return halide_error_code_copy_to_device_failed;
+    }
+    j.up_sent = j.up_total;
+    j.up_recorded = true;
+    return 0;
+}
+
+// A larger job's phase history, then its pixel locations, in chunks of kUploadChunkBytes on the
+// stage stream, at most kUploadChunksAhead outstanding; ev_up after the last one.
+int advance_upload(sarbp_server *s, Job &j) {
+    hipStream_t st = s->stage;
+    while (j.chunks_done < j.chunks_enqueued &&
+           event_done(j.ev_chunk[j.chunks_done % kUploadChunksAhead]))
+        ++j.chunks_done;
+    const size_t b_phs = 8 * (size_t)j.d.npulses * (size_t)j.d.nsamples;
+    hipError_t e = hipSuccess;
+    while (j.up_sent < j.up_total && j.chunks_enqueued - j.chunks_done < kUploadChunksAhead) {
+        hipEvent_t &ev = j.ev_chunk[j.chunks_enqueued % kUploadChunksAhead];
+        if (!ev && (e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) break;
+        // a chunk stays within one of the two arrays
+        const bool in_phs = j.up_sent < b_phs;
+        const size_t at = in_phs ? j.up_sent : j.up_sent - b_phs;
+        const size_t n = std::min(kUploadChunkBytes, (in_phs ? b_phs : j.up_total - b_phs) - at);
+        uint8_t *dst = j.dev + (in_phs ? j.ar.phs : j.ar.r) + at;
+        const uint8_t *src = (const uint8_t *)(in_phs ? j.d.phs : (const void *)j.d.pixel_locs) + at;
+        if ((e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st)) != hipSuccess ||
+            (e = hipEventRecord(ev, st)) != hipSuccess)
+            break;
+        j.up_sent += n;
+        ++j.chunks_enqueued;
+    }
+    if (e == hipSuccess && j.up_sent == j.up_total && !j.up_recorded) {
+        if ((e = hipEventRecord(j.ev_up, st)) == hipSuccess) j.up_recorded = true;
+    }
     if (e != hipSuccess) {
         set_error("job %llu: upload failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
         return halide_error_code_copy_to_device_failed;
@@ -401,16 +461,48 @@ int prepare_job(sarbp_server *s, Job &j, hipStream_t st) {
     return 0;
 }
 
-// Output download of an accumulated job and its completion event, on the job's download stream.
-// On failure nothing is recorded.
+// Output download of an accumulated job and its completion event, on the job's download stream:
+// one copy, or a larger job's output of more than two chunks by advance_download (the copies of both
+// directions share the device's copy engines, so that a small job's upload or download waits for
+// at most kUploadChunksAhead chunks of a large one). On failure nothing is recorded.
 int finish_job(sarbp_server *s, Job &j) {
-    hipStream_t st = j.small ? s->dl_small : s->dl_big;
+    j.phase = Phase::Downloading;
     const size_t npix = (size_t)j.d.nu * (size_t)j.d.nv;
-    hipError_t e = hipMemcpyAsync(j.d.out, j.dev + j.ar.out, 16 * npix, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && (e = hipEventRecord(j.ev_done, st)) == hipSuccess) {
-        j.done_recorded = true;
-        j.phase = Phase::Downloading;
+    if (!j.small && 16 * npix > kUploadChunkBytes * kUploadChunksAhead) {
+        j.dl_sent = 0;
+        j.chunks_enqueued = j.chunks_done = 0;  // the upload's chunks completed before
+        return advance_download(s, j);
     }
+    hipStream_t st = j.small ? s->dl_small : s->dl_big;
+    hipError_t e = hipMemcpyAsync(j.d.out, j.dev + j.ar.out, 16 * npix, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && (e = hipEventRecord(j.ev_done, st)) == hipSuccess) j.done_recorded = true;
+    if (e != hipSuccess) {
+        set_error("job %llu: download failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
+        return halide_error_code_copy_to_host_failed;
+    }
+    return 0;
+}
+
+int advance_download(sarbp_server *s, Job &j) {
+    hipStream_t st = s->dl_big;
+    while (j.chunks_done < j.chunks_enqueued &&
+           event_done(j.ev_chunk[j.chunks_done % kUploadChunksAhead]))
+        ++j.chunks_done;
+    const size_t total = 16 * (size_t)j.d.nu * (size_t)j.d.nv;
+    hipError_t e = hipSuccess;
+    while (j.dl_sent < total && j.chunks_enqueued - j.chunks_done < kUploadChunksAhead) {
+        hipEvent_t &ev = j.ev_chunk[j.chunks_enqueued % kUploadChunksAhead];
+        if (!ev && (e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) break;
+        const size_t n = std::min(kUploadChunkBytes, total - j.dl_sent);
+        if ((e = hipMemcpyAsync((uint8_t *)j.d.out + j.dl_sent, j.dev + j.ar.out + j.dl_sent, n,
+                                hipMemcpyDeviceToHost, st)) != hipSuccess ||
+            (e = hipEventRecord(ev, st)) != hipSuccess)
+            break;
+        j.dl_sent += n;
+        ++j.chunks_enqueued;
+    }
+    if (e == hipSuccess && j.dl_sent == total && (e = hipEventRecord(j.ev_done, st)) == hipSuccess)
+        j.done_recorded = true;
     if (e != hipSuccess) {
         set_error("job %llu: download failed: %s", (unsigned long long)j.id, hipGetErrorString(e));
         return halide_error_code_copy_to_host_failed;
@@ -431,20 +523,37 @@ void fail_inflight(sarbp_server *s, Job &j, int code) {
 // the image, so the result is the same bit for bit. At most kBandsAhead bands are enqueued at a
 // time; the scheduler adds the next band when the oldest completes, so work on any stream that
 // shares a hardware queue with this job's stream waits for at most kBandsAhead bands.
-int band_rows(const BpArgs &a) {
+int band_rows(const sarbp_server *s, const BpArgs &a, bool shared) {
     const int64_t cols = (a.width + 15) / 16, th = tile_height(a.ppt);
-    return (int)(std::max<int64_t>(1, kBandPixels / (16 * th * cols)) * th);
+    int64_t tile_rows = std::max<int64_t>(1, kBandPixels / (16 * th * cols));
+    if (shared) {
+        const int64_t wave = accumulate_resident_tiles(s->cfg.device, a.q_format, a.ppt, a.g.on != 0);
+        const int64_t by_pp = kBandPixelPulsesShared / ((int64_t)a.np * 16 * th * cols);
+        tile_rows = std::min(tile_rows, std::max<int64_t>({1, (wave + cols - 1) / cols, by_pp}));
+    }
+    return (int)(tile_rows * th);
 }
 
-// Enqueue bands while fewer than kBandsAhead are outstanding; after the last band, ev_acc.
+// Whether a small job is admitted and not yet accumulated (its upload, range profiles or fused
+// launch still to run).
+bool small_jobs_active(const sarbp_server *s) {
+    for (const auto &jp : s->inflight)
+        if (jp->small && !jp->done_recorded && jp->phase != Phase::Downloading) return true;
+    return false;
+}
+
+// Enqueue bands while fewer than kBandsAhead (one while small jobs are active) are outstanding;
+// after the last band, ev_acc.
 int advance_bands(sarbp_server *s, Job &j) {
     hipStream_t st = s->streams[j.stream];
     while (j.bands_done < j.bands_enqueued &&
            hipEventQuery(j.ev_band[j.bands_done % kBandsAhead]) == hipSuccess)
         ++j.bands_done;
-    const int rows = band_rows(j.args);
+    const bool shared = small_jobs_active(s);
+    const int ahead = shared ? 1 : kBandsAhead;
+    const int rows = band_rows(s, j.args, shared);
     hipError_t e = hipSuccess;
-    while (j.next_row < j.args.height && j.bands_enqueued - j.bands_done < kBandsAhead) {
+    while (j.next_row < j.args.height && j.bands_enqueued - j.bands_done < ahead) {
         BpArgs a = j.args;
         a.y0 = j.args.y0 + j.next_row;
         a.height = std::min(rows, j.args.height - j.next_row);
@@ -514,6 +623,10 @@ int run_alone(sarbp_server *s, Job &j) {
     }
     j.args.ppt = j.ppt;
     j.args.row_windows = j.row_windows;
+    // bands follow the range profiles on the job's stream (a small job sent here from the fused
+    // path was prepared before)
+    j.t_launch = now_ns(s->t0);
+    if (!j.t_ready) j.t_ready = j.t_launch;
     j.banding = true;
     j.next_row = 0;
     for (int k = 0; k < kBandsAhead; ++k)
@@ -562,9 +675,11 @@ void run_fused(sarbp_server *s, FuseSlot &slot, std::vector<Job *> &group) {
     }
     slot.busy = true;
     slot.group = group;
+    const int64_t t_launch = now_ns(s->t0);
     for (Job *j : group) {
         j->fused = true;
         j->phase = Phase::Computing;
+        j->t_launch = t_launch;
     }
 }
 
@@ -581,6 +696,8 @@ void retire(sarbp_server *s, Job &j) {
     (void)hipEventDestroy(j.ev_begin);
     for (hipEvent_t &ev : j.ev_band)
         if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t &ev : j.ev_chunk)
+        if (ev) (void)hipEventDestroy(ev);
 }
 
 // Pulse table and range profiles of an uploaded small job on the prep stream, then ev_ready.
@@ -594,8 +711,6 @@ int prep_small(sarbp_server *s, Job &j) {
     j.phase = Phase::Prepping;
     return 0;
 }
-
-bool event_done(hipEvent_t ev) { return hipEventQuery(ev) == hipSuccess; }
 
 void scheduler(sarbp_server *s) {
     (void)hipSetDevice(s->cfg.device);
@@ -619,7 +734,9 @@ void scheduler(sarbp_server *s) {
         for (FuseSlot &sl : s->slots) {
             if (!sl.busy || !event_done(sl.ev)) continue;
             sl.busy = false;
+            const int64_t t_acc = now_ns(s->t0);
             for (Job *j : sl.group) {
+                j->t_acc = t_acc;
                 int rv = finish_job(s, *j);
                 if (rv) fail_inflight(s, *j, rv);
             }
@@ -629,16 +746,25 @@ void scheduler(sarbp_server *s) {
             Job &j = *jp;
             if (j.done_recorded) continue;
             int rv = 0;
-            if (j.phase == Phase::Uploading && event_done(j.ev_up)) {
+            if (j.phase == Phase::Downloading) {
+                rv = advance_download(s, j);  // a larger job's output, chunk by chunk
+            } else if (j.phase == Phase::Uploading && !j.up_recorded) {
+                rv = advance_upload(s, j);
+            } else if (j.phase == Phase::Uploading && event_done(j.ev_up)) {
+                j.t_up = now_ns(s->t0);
                 rv = j.small ? prep_small(s, j) : run_alone(s, j);
             } else if (j.phase == Phase::Prepping && event_done(j.ev_ready)) {
+                j.t_ready = now_ns(s->t0);
                 j.phase = Phase::Ready;
                 s->fuse_pending.push_back(&j);
             } else if (j.phase == Phase::Computing && j.banding) {
                 rv = advance_bands(s, j);
                 if (rv) j.banding = false;
             }
-            if (rv == 0 && j.phase == Phase::Finishing && event_done(j.ev_acc)) rv = finish_job(s, j);
+            if (rv == 0 && j.phase == Phase::Finishing && event_done(j.ev_acc)) {
+                j.t_acc = now_ns(s->t0);
+                rv = finish_job(s, j);
+            }
             if (rv) fail_inflight(s, j, rv);
         }
         for (auto &kv : s->tuning) {
@@ -937,6 +1063,13 @@ extern "C" int sarbp_server_wait(sarbp_server *s, uint64_t job_id, int32_t timeo
         stats->status = j->status;
         stats->host_enqueue_us = (int32_t)std::min<int64_t>(j->host_ns / 1000, INT32_MAX);
         stats->device_us = (int32_t)std::min<double>(1000.0 * j->device_ms, (double)INT32_MAX);
+        auto since_admit = [&](int64_t t) {
+            return t ? (int32_t)std::min<int64_t>((t - j->t_admit) / 1000, INT32_MAX) : -1;
+        };
+        stats->up_us = since_admit(j->t_up);
+        stats->ready_us = since_admit(j->t_ready);
+        stats->launch_us = since_admit(j->t_launch);
+        stats->acc_us = since_admit(j->t_acc);
     }
     s->jobs.erase(job_id);
     if (j->status < 0) set_error("job %llu failed (%d)", (unsigned long long)job_id, j->status);

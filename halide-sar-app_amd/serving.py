@@ -65,7 +65,8 @@ class JobStats(ctypes.Structure):
     _fields_ = [("t_submit_ns", _i64), ("t_admit_ns", _i64), ("t_end_ns", _i64),
                 ("device_bytes", _i64), ("fused", _i32), ("tile_ppt", _i32), ("status", _i32),
                 ("host_enqueue_us", _i32), ("device_us", _i32), ("row_windows", _i32),
-                ("reserved", _i32 * 2)]
+                ("reserved", _i32 * 2), ("up_us", _i32), ("ready_us", _i32),
+                ("launch_us", _i32), ("acc_us", _i32)]
 
 
 _lib.sarbp_server_config_init.argtypes = [ctypes.POINTER(ServerConfig)]
@@ -293,7 +294,8 @@ def replay(server: Server, factory: JobFactory, rate_hz: float | None, keep_outp
             try:
                 st = server.wait(ids[i])
                 done[i] = (st.t_submit_ns, st.t_admit_ns, st.t_end_ns, st.fused, st.tile_ppt,
-                           st.host_enqueue_us, st.device_us, st.row_windows)
+                           st.host_enqueue_us, st.device_us, st.row_windows,
+                           (st.up_us, st.ready_us, st.launch_us, st.acc_us))
             except Exception as e:  # noqa: BLE001 - reported after the stream
                 errors.append((i, str(e)))
             if i in keep:
@@ -320,16 +322,28 @@ def replay(server: Server, factory: JobFactory, rate_hz: float | None, keep_outp
         raise RuntimeError(f"{len(errors)} jobs failed, first: {errors[0]}")
     recs = []
     for i, sp in enumerate(factory.specs):
-        ts, ta, te, fused, ppt, hu, du, rw = done[i]
+        ts, ta, te, fused, ppt, hu, du, rw, steps = done[i]
         recs.append(dict(latency_ms=(te - ts) / 1e6, queue_ms=(ta - ts) / 1e6, t_end_ms=te / 1e6,
                          t_submit_ms=ts / 1e6, fused=fused, ppt=ppt, row_windows=rw,
                          host_enqueue_ms=hu / 1e3,
-                         device_ms=du / 1e3,
+                         device_ms=du / 1e3, steps_ms=_step_ms(steps, (te - ta) / 1e6),
                          pp=factory.pixel_pulses(i), big=sp.get("big", False)))
     return recs, wall, kept
 
 
 SMALL_JOB_PIXEL_PULSES = 2 ** 31  # the engine's default fusion bound: jobs at most this size are "small"
+STEPS = ("upload", "profiles", "wait", "accumulate", "download")
+
+
+def _step_ms(steps_us, admitted_ms):
+    """Durations of a job's steps after admission from the engine's step times (microseconds after
+    admission): upload, range profiles, wait for its accumulation launch, accumulation, download
+    (to the end of the job). A job accumulated in bands enqueues its range profiles with its bands,
+    so its profiles and wait steps are 0 and its accumulation includes the profiles."""
+    up, ready, launch, acc = (t / 1e3 for t in steps_us)
+    if min(up, ready, launch, acc) < 0:
+        return None
+    return dict(zip(STEPS, (up, ready - up, launch - ready, acc - launch, admitted_ms - acc)))
 
 
 def summarize(recs, wall, alone_ms=None) -> dict:
@@ -364,6 +378,11 @@ def _by_class(recs, lat, alone_ms) -> dict:
             out[f"{name}_jobs"] = int(sel.sum())
             out[f"{name}_p50_ms"] = float(np.percentile(lat[sel], 50))
             out[f"{name}_p99_ms"] = float(np.percentile(lat[sel], 99))
+        recs_c = [r for r, m in zip(recs, sel) if m and r.get("steps_ms")]
+        if recs_c:  # p50 and p99 of each step's duration
+            out[f"{name}_steps_ms"] = {
+                k: [round(float(np.percentile([r["steps_ms"][k] for r in recs_c], q)), 3)
+                    for q in (50, 99)] for k in STEPS}
     if alone_ms is not None:
         slow = lat / np.maximum(np.asarray(alone_ms, np.float64), 1e-3)
         out["slowdown_p50"] = float(np.percentile(slow, 50))

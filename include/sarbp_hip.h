@@ -314,6 +314,11 @@ typedef struct sarbp_job_stats {
                                                   its output in host memory */
     int32_t row_windows;                       /* bands staged row windows in LDS */
     int32_t reserved[2];
+    /* When the scheduler saw each step of the job end or began it, in microseconds after
+     * t_admit_ns (-1: not reached): inputs uploaded, range profiles computed (a job accumulated
+     * in bands: when its range profiles and bands were enqueued together), accumulation enqueued
+     * (its fused launch or first band), accumulation complete. The scheduler polls every 20 us. */
+    int32_t up_us, ready_us, launch_us, acc_us;
 } sarbp_job_stats;
 
 void sarbp_server_config_init(sarbp_server_config *cfg);

@@ -18,7 +18,7 @@
 //            loaded of num_streams streams)
 //   fuse     accumulation of small jobs: prepared small jobs gather while every launch-table slot
 //            is in flight, and when one is free, they are computed by one launch over a
-//            job-indexed tile list (up to the table's capacity and kFusePixelPulses)
+//            job-indexed tile list (up to the table's capacity and fuse_launch_pixel_pulses)
 //   dl_small, dl_big   output downloads to the caller's buffers
 // A step that depends on another stream's work is enqueued by the scheduler once it has seen that
 // work complete (event query), never as a device-side wait: streams share the process's few
@@ -61,16 +61,15 @@ constexpr int kFuseSlots = 2;                       // fused launches in flight 
 // holds this many pixels whatever the tile height: 4096 tiles of 16 x 32 (two device waves at 8
 // tiles per CU) or 2048 of 16 x 64 (two waves at 4 per CU), so that its duration does not grow with
 // the tile the tuner picks.
-// While small jobs are in flight, bands are shorter (kBandPixelPulsesShared, but at least one
-// device wave of tiles) and only one is enqueued ahead, so that a small job's kernels wait for at
-// most about one short band.
 constexpr int64_t kBandPixels = 4096 * 16 * 32;
-constexpr int64_t kBandPixelPulsesShared = 4'000'000'000LL;
 constexpr int kBandsAhead = 2;
+// Four chunks in flight (~0.6 ms of copying) keep the copy engine busy across a scheduler pass
+// over many jobs; with two, a large job's output took 2-5x as long when 2000 jobs were in flight.
 constexpr size_t kUploadChunkBytes = (size_t)8 << 20;  // ~0.15 ms over PCIe 5 x16
-constexpr int kUploadChunksAhead = 2;
-// Pixel-pulses per fused launch at most (~6 ms of accumulation on an idle device), so that a fused
-// launch holds the hardware queue it shares with other streams no longer than a band does.
+constexpr int kUploadChunksAhead = 4;
+// Pixel-pulses per fused launch at most by default (~4 ms of accumulation on an idle device), so
+// that a fused launch holds the device no longer than a band does (sarbp_server_config.
+// fuse_launch_pixel_pulses).
 constexpr int64_t kFusePixelPulses = 8'000'000'000LL;
 // Blocks of the range-profile launches of the engine (256 threads each, looping over pulses).
 constexpr int kSharedFftBlocks = 512;
@@ -462,7 +461,8 @@ int prepare_job(sarbp_server *s, Job &j, hipStream_t st) {
 }
 
 // Output download of an accumulated job and its completion event, on the job's download stream:
-// one copy, or a larger job's output of more than two chunks by advance_download (the copies of both
+// one copy, or a larger job's output of more than kUploadChunksAhead chunks by advance_download
+// (the copies of both
 // directions share the device's copy engines, so that a small job's upload or download waits for
 // at most kUploadChunksAhead chunks of a large one). On failure nothing is recorded.
 int finish_job(sarbp_server *s, Job &j) {
@@ -523,37 +523,20 @@ void fail_inflight(sarbp_server *s, Job &j, int code) {
 // the image, so the result is the same bit for bit. At most kBandsAhead bands are enqueued at a
 // time; the scheduler adds the next band when the oldest completes, so work on any stream that
 // shares a hardware queue with this job's stream waits for at most kBandsAhead bands.
-int band_rows(const sarbp_server *s, const BpArgs &a, bool shared) {
+int band_rows(const BpArgs &a) {
     const int64_t cols = (a.width + 15) / 16, th = tile_height(a.ppt);
-    int64_t tile_rows = std::max<int64_t>(1, kBandPixels / (16 * th * cols));
-    if (shared) {
-        const int64_t wave = accumulate_resident_tiles(s->cfg.device, a.q_format, a.ppt, a.g.on != 0);
-        const int64_t by_pp = kBandPixelPulsesShared / ((int64_t)a.np * 16 * th * cols);
-        tile_rows = std::min(tile_rows, std::max<int64_t>({1, (wave + cols - 1) / cols, by_pp}));
-    }
-    return (int)(tile_rows * th);
+    return (int)(std::max<int64_t>(1, kBandPixels / (16 * th * cols)) * th);
 }
 
-// Whether a small job is admitted and not yet accumulated (its upload, range profiles or fused
-// launch still to run).
-bool small_jobs_active(const sarbp_server *s) {
-    for (const auto &jp : s->inflight)
-        if (jp->small && !jp->done_recorded && jp->phase != Phase::Downloading) return true;
-    return false;
-}
-
-// Enqueue bands while fewer than kBandsAhead (one while small jobs are active) are outstanding;
-// after the last band, ev_acc.
+// Enqueue bands while fewer than kBandsAhead are outstanding; after the last band, ev_acc.
 int advance_bands(sarbp_server *s, Job &j) {
     hipStream_t st = s->streams[j.stream];
     while (j.bands_done < j.bands_enqueued &&
            hipEventQuery(j.ev_band[j.bands_done % kBandsAhead]) == hipSuccess)
         ++j.bands_done;
-    const bool shared = small_jobs_active(s);
-    const int ahead = shared ? 1 : kBandsAhead;
-    const int rows = band_rows(s, j.args, shared);
+    const int rows = band_rows(j.args);
     hipError_t e = hipSuccess;
-    while (j.next_row < j.args.height && j.bands_enqueued - j.bands_done < ahead) {
+    while (j.next_row < j.args.height && j.bands_enqueued - j.bands_done < kBandsAhead) {
         BpArgs a = j.args;
         a.y0 = j.args.y0 + j.next_row;
         a.height = std::min(rows, j.args.height - j.next_row);
@@ -837,7 +820,7 @@ void scheduler(sarbp_server *s) {
 
         // 3. small jobs: while every launch-table slot is in flight, prepared jobs wait and gather;
         // when a slot is free, the waiting jobs go into one launch per (format, tile shape, grid
-        // mode), as many as the slot's table and kFusePixelPulses allow. An idle GPU thus starts a
+        // mode), as many as the slot's table and fuse_launch_pixel_pulses allow. An idle GPU thus starts a
         // small job at once and a busy one receives them in large batches.
         for (FuseSlot &sl : s->slots) {
             if (s->fuse_pending.empty()) break;
@@ -852,7 +835,7 @@ void scheduler(sarbp_server *s) {
                 const size_t need = sizeof(BpArgs) + 256 + sizeof(FusedTile) * tiles_of(*j, th);
                 if (j->d.q_format == fmt && j->ppt == ppt && j->args.g.on == grid &&
                     (group.empty() ||
-                     (bytes + need <= kFuseSlotBytes && pp + j->pp <= kFusePixelPulses))) {
+                     (bytes + need <= kFuseSlotBytes && pp + j->pp <= s->cfg.fuse_launch_pixel_pulses))) {
                     group.push_back(j);
                     bytes += need;
                     pp += j->pp;
@@ -916,6 +899,7 @@ extern "C" void sarbp_server_config_init(sarbp_server_config *c) {
     c->fuse_max_pixel_pulses = (int64_t)1 << 31;
     c->num_streams = 2;  // streams for large jobs, besides the stage, prep, fuse and download ones
     c->autotune = 1;
+    c->fuse_launch_pixel_pulses = kFusePixelPulses;
 }
 
 extern "C" int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out) {
@@ -936,6 +920,7 @@ extern "C" int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server 
     }
     auto s = std::make_unique<sarbp_server>();
     s->cfg = *cfg;
+    if (s->cfg.fuse_launch_pixel_pulses <= 0) s->cfg.fuse_launch_pixel_pulses = kFusePixelPulses;
     s->stream_load.assign((size_t)cfg->num_streams, 0);
     s->t0 = std::chrono::steady_clock::now();
     int prev = 0;

@@ -49,7 +49,7 @@ _P, _i32, _i64, _f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", _i32), ("max_inflight", _i32), ("hbm_budget_bytes", _i64),
                 ("fuse_max_pixel_pulses", _i64), ("num_streams", _i32), ("autotune", _i32),
-                ("reserved", _i32 * 8)]
+                ("fuse_launch_pixel_pulses", _i64), ("reserved", _i32 * 6)]
 
 
 class JobDesc(ctypes.Structure):
@@ -85,13 +85,15 @@ class ServingError(RuntimeError):
 
 class Server:
     def __init__(self, device=0, max_inflight=64, hbm_budget_gb=64.0, fuse_max_pixel_pulses=2 ** 31,
-                 num_streams=2, autotune=True):
+                 num_streams=2, autotune=True, fuse_launch_pixel_pulses=None):
         c = ServerConfig()
         _lib.sarbp_server_config_init(ctypes.byref(c))
         c.device, c.max_inflight = device, max_inflight
         c.hbm_budget_bytes = int(hbm_budget_gb * (1 << 30))
         c.fuse_max_pixel_pulses = int(fuse_max_pixel_pulses)
         c.num_streams, c.autotune = num_streams, int(bool(autotune))
+        if fuse_launch_pixel_pulses is not None:
+            c.fuse_launch_pixel_pulses = int(fuse_launch_pixel_pulses)
         self.cfg = c
         self._s = _P()
         rv = _lib.sarbp_server_create(ctypes.byref(c), ctypes.byref(self._s))

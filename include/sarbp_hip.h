@@ -279,7 +279,10 @@ typedef struct sarbp_server_config {
     int64_t fuse_max_pixel_pulses;  /* jobs with nu*nv*npulses <= this are fused (0: never) */
     int32_t num_streams;            /* large-job streams, 1..16 (default 2) */
     int32_t autotune;               /* 1: time the tile candidates on a shape class's first job */
-    int32_t reserved[8];
+    int64_t fuse_launch_pixel_pulses; /* pixel-pulses of one fused launch at most, which bounds
+                                         how long other jobs' kernels wait behind one (0: the
+                                         default, 8e9, ~4 ms on an idle device) */
+    int32_t reserved[6];
 } sarbp_server_config;
 
 typedef struct sarbp_job {

@@ -8,6 +8,7 @@
 // pixel locations, the accumulation, downloads and the scatter of different bands overlap; bands
 // alternate between two compute streams so that one band's last wave overlaps the next band.
 #include <hip/hip_runtime.h>
+#include <sched.h>
 
 #include <algorithm>
 #include <cmath>
@@ -145,7 +146,7 @@ int get_context(Context **out) {
 // callers on several host threads may share the pool.
 class HostPool {
   public:
-    static constexpr size_t kMaxThreads = 8;
+    static constexpr size_t kMaxThreads = 16;
 
     static HostPool &get() {
         static HostPool pool;
@@ -221,10 +222,20 @@ class HostPool {
 };
 
 // Pieces for n bytes of host work: one per 2 MB, at most the pool's threads and the host's cores.
+// CPUs this process may run on (its affinity mask; the machine's count where that is unknown).
+size_t usable_cpus() {
+    static const size_t n = [] {
+        cpu_set_t set;
+        if (sched_getaffinity(0, sizeof set, &set) == 0 && CPU_COUNT(&set) > 0)
+            return (size_t)CPU_COUNT(&set);
+        return (size_t)std::max(1u, std::thread::hardware_concurrency());
+    }();
+    return n;
+}
+
 size_t host_parts(size_t n) {
     constexpr size_t kMin = (size_t)2 << 20;
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    return std::max<size_t>(1, std::min<size_t>({HostPool::kMaxThreads, hw, n / kMin}));
+    return std::max<size_t>(1, std::min<size_t>({HostPool::kMaxThreads, usable_cpus(), n / kMin}));
 }
 
 void par_copy(void *dst, const void *src, size_t n) {
@@ -554,19 +565,42 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
     // r = constant_exterior(r_in, 0.0) (backprojection.cpp:77): a band reads r only at its own pixels,
     // px = nu * (nv - y - 1) + x (backprojection.cpp:130), a contiguous index range per band.
     const double *rs = (const double *)r->host;
+    // band b's pixel range [plo, phi), and within r's extent [lo, hi)
+    auto band_pixels = [&](int b, int64_t &lo, int64_t &hi, int64_t *plo = nullptr,
+                           int64_t *phi = nullptr) {
+        const int row0 = b * band, nrows = std::min(band, oh - row0);
+        const int64_t ylo = oy0 + row0, yhi = ylo + nrows - 1;
+        const int64_t pl = (int64_t)nu * (nv - yhi - 1) + ox0, ph = (int64_t)nu * (nv - ylo - 1) + ox0 + ow;
+        lo = std::max<int64_t>(rmin, pl);
+        hi = std::min<int64_t>(rmin + rext, ph);
+        if (plo) *plo = pl;
+        if (phi) *phi = ph;
+    };
+    // Whether band b is on the image plane (checked once, in band order). A band with pixels
+    // outside r's extent reads r (0 there); once one band is off the plane, the rest read r
+    // unchecked.
+    std::vector<signed char> grid_band((size_t)nbands, -1);
+    auto band_grid = [&](int b) {
+        if (grid_band[b] < 0) {
+            int64_t lo, hi, plo, phi;
+            band_pixels(b, lo, hi, &plo, &phi);
+            plane.on = plane.on && plo >= rmin && phi <= rmin + rext && band_on_plane(plane, lo, hi);
+            grid_band[b] = plane.on ? 1 : 0;
+        }
+        return grid_band[b] == 1;
+    };
     g_last_grid_bands = 0;
     g_last_bands = nbands;
     for (int b = 0; b < nbands; ++b) {
         const int row0 = b * band, nrows = std::min(band, oh - row0);
-        const int64_t ylo = oy0 + row0, yhi = ylo + nrows - 1;
-        const int64_t plo = (int64_t)nu * (nv - yhi - 1) + ox0;
-        const int64_t phi = (int64_t)nu * (nv - ylo - 1) + ox0 + ow;
-        const int64_t lo = std::max<int64_t>(rmin, plo), hi = std::min<int64_t>(rmin + rext, phi);
+        int64_t lo, hi;
+        band_pixels(b, lo, hi);
         hipStream_t sc = (b & 1) ? s3 : s1;
-        // a band on the image plane: no pixel locations to move. A band with pixels outside r's
-        // extent reads r (0 there); once one band is off the plane, the rest read r unchecked.
-        if (plane.on &&
-            (plane.on = plo >= rmin && phi <= rmin + rext && band_on_plane(plane, lo, hi))) {
+        // The first two bands are both checked before the first is launched, so that they run
+        // side by side from the start (a band is half a device of tiles) and the pairs after them
+        // stay aligned to the last band.
+        if (b == 0 && nbands > 1) (void)band_grid(0), (void)band_grid(1);
+        if (band_grid(b)) {
             ++g_last_grid_bands;
             rv = plan_accumulate_rows(plan, dr, rmin, rext, row0, nrows,
                                       (double *)ctx->d_out.p + (size_t)2 * ow * row0, sc, true);

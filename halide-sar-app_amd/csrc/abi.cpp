@@ -5,8 +5,9 @@
 // answers bounds queries, packs the host inputs into pinned staging memory, runs the pipeline on the
 // GPU and scatters the result into the caller's (possibly strided, possibly partial) output.
 // backprojection() splits the output into row bands so that host packing, uploads of each band's
-// pixel locations, the accumulation, downloads and the scatter of different bands overlap; bands
-// alternate between two compute streams so that one band's last wave overlaps the next band.
+// pixel locations, the accumulation (which writes each band into page-locked host memory) and the
+// scatter of different bands overlap; bands alternate between two compute streams so that one
+// band's last wave overlaps the next band.
 #include <hip/hip_runtime.h>
 #include <sched.h>
 
@@ -62,11 +63,15 @@ void set_dense(halide_buffer_t *b, std::initializer_list<std::pair<int, int>> mi
     }
 }
 
-// Growable device / pinned-host buffers of one device.
+// Growable device / pinned-host buffers of one device. Pinned buffers take hipHostMalloc flags.
+#ifndef SARBP_OUT_HOST_FLAGS
+#define SARBP_OUT_HOST_FLAGS hipHostMallocCoherent
+#endif
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
     bool pinned = false;
+    unsigned host_flags = hipHostMallocDefault;
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap) return hipSuccess;
         if (p) {
@@ -76,7 +81,7 @@ struct DevBuf {
             cap = 0;
         }
         size_t want = bytes + bytes / 8;
-        hipError_t e = pinned ? hipHostMalloc(&p, want, hipHostMallocDefault) : hipMalloc(&p, want);
+        hipError_t e = pinned ? hipHostMalloc(&p, want, host_flags) : hipMalloc(&p, want);
         if (e == hipSuccess) cap = want;
         else p = nullptr;
         return e;
@@ -88,7 +93,7 @@ struct Context {
     int device = 0;
     hipStream_t stream = nullptr;   // computation (range profiles, even bands)
     hipStream_t stream3 = nullptr;  // computation (odd bands)
-    hipStream_t stream2 = nullptr;  // band uploads and downloads
+    hipStream_t stream2 = nullptr;  // uploads of the bands' pixel locations
     std::vector<hipEvent_t> events;
     hipError_t ensure_events(size_t n) {
         while (events.size() < n) {
@@ -99,8 +104,12 @@ struct Context {
         }
         return hipSuccess;
     }
-    DevBuf d_phs, d_kr, d_pos, d_r, d_out, d_aux;
-    DevBuf h_in{nullptr, 0, true}, h_out{nullptr, 0, true};
+    DevBuf d_aux, d_out;  // device staging (d_out: img_output_*)
+    // Inputs are packed into h_in and uploaded. The accumulation writes each band straight into
+    // h_out (page-locked, mapped into the device's address space; fine-grained, so the writes
+    // leave as the kernel makes them): no download, and no copy kernel competing with the next
+    // band for compute units.
+    DevBuf h_in{nullptr, 0, true}, h_out{nullptr, 0, true, SARBP_OUT_HOST_FLAGS};
     std::map<std::tuple<int, int, int, int, int, int, int, int, int, int, int, int, long long,
                         long long, double>,
              sarbp_plan *>
@@ -265,6 +274,7 @@ struct PlaneGuess {
     bool on = false;
     double d_u = 0.0, d_v = 0.0, uh[3] = {}, vh[3] = {}, tol = 0.0;
     std::vector<double> u, v;
+    std::vector<double> uu[3];  // u_hat[c] u[x], the plane's first product
     const double *rc[3] = {};  // pixel i's component c at rc[c][i - rmin]
     int64_t rmin = 0;
     int nu = 0;
@@ -309,13 +319,19 @@ void guess_plane(const halide_buffer_t *u, const halide_buffer_t *v, const halid
         g.vh[c] = g.rc[c][ib - rmin] / g.v[ya];  // r(x0, ya) = v_hat v[ya]
         if (!std::isfinite(g.uh[c]) || !std::isfinite(g.vh[c])) return;
     }
+    for (int c = 0; c < 3; ++c) {
+        g.uu[c].resize((size_t)nu);
+        for (int x = 0; x < nu; ++x) g.uu[c][x] = g.uh[c] * g.u[x];
+    }
     g.tol = std::ldexp(umax + vmax, -40);
     g.rmin = rmin;
     g.nu = nu;
     g.on = true;
 }
 
-// Whether pixels [lo, hi) of r lie within g.tol of the plane (checked on the host pool).
+// Whether pixels [lo, hi) of r lie within g.tol of the plane (checked on the host pool), one row
+// segment at a time: m = (0 + u_hat u[x]) + v_hat v[y] with the row's product taken once and the
+// first from g.uu, the same roundings as per pixel; a NaN location fails.
 bool band_on_plane(const PlaneGuess &g, int64_t lo, int64_t hi) {
     if (hi <= lo) return true;
     const size_t parts = host_parts((size_t)(hi - lo) * 24);
@@ -323,15 +339,20 @@ bool band_on_plane(const PlaneGuess &g, int64_t lo, int64_t hi) {
     std::vector<char> ok(parts, 1);
     HostPool::get().run(parts, [&](size_t k) {
         const int64_t a = lo + (int64_t)k * chunk, b = std::min(hi, a + chunk);
-        for (int64_t i = a; i < b; ++i) {
-            const int64_t y = i / g.nu, x = i - y * g.nu;
+        for (int64_t i = a; i < b;) {
+            const int64_t y = i / g.nu, x0 = i - y * g.nu, n = std::min(b - i, g.nu - x0);
+            int64_t bad = 0;
             for (int c = 0; c < 3; ++c) {
-                const double m = (0.0 + g.uh[c] * g.u[x]) + g.vh[c] * g.v[y];
-                if (!(std::fabs(g.rc[c][i - g.rmin] - m) <= g.tol)) {
-                    ok[k] = 0;
-                    return;
-                }
+                const double vv = g.vh[c] * g.v[y], tol = g.tol;
+                const double *uu = g.uu[c].data() + x0, *rr = g.rc[c] + (i - g.rmin);
+                for (int64_t t = 0; t < n; ++t)
+                    bad += !(std::fabs(rr[t] - ((0.0 + uu[t]) + vv)) <= tol);
             }
+            if (bad) {
+                ok[k] = 0;
+                return;
+            }
+            i += n;
         }
     });
     for (char o : ok)
@@ -471,8 +492,7 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
     hipError_t e = ctx->h_in.ensure(b_phs + b_kr + b_pos + b_r);
     if (e == hipSuccess) e = ctx->h_out.ensure(b_out);
     if (e != hipSuccess) return hip_fail(e, halide_error_code_out_of_memory, "pinned staging");
-    if ((e = ctx->d_aux.ensure(b_phs + b_kr + b_pos + b_r)) != hipSuccess ||
-        (e = ctx->d_out.ensure(b_out)) != hipSuccess)
+    if ((e = ctx->d_aux.ensure(b_phs + b_kr + b_pos + b_r)) != hipSuccess)
         return hip_fail(e, halide_error_code_device_malloc_failed, "device staging");
     uint8_t *h = (uint8_t *)ctx->h_in.p;
     float *hphs = (float *)h;
@@ -546,7 +566,8 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
     // 1024 tiles at 16 x 32, 512 at 16 x 64), tile-aligned, at most 8, alternating between the two
     // compute streams. Against bands of a whole device wave the first band waits for half as many
     // pixel locations and the last one runs alone for half as long (C2, 16 x 32 tiles: 8.6-9.1 ms per
-    // call instead of 9.0-9.4).
+    // call instead of 9.0-9.4). Bands over three streams in turn measured slower (C2: 6.46 against
+    // 6.02 ms per call).
     const int th = plan_tile_rows(plan);
     const int64_t tile_rows = (oh + th - 1) / th, tile_cols = (ow + 15) / 16;
     const int64_t band_tiles = std::max(256, plan_resident_tiles(plan) / 2);
@@ -554,10 +575,10 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
         1, std::min<int64_t>({8, tile_rows, tile_rows * tile_cols / band_tiles}));
     const int band = (int)((tile_rows + nb - 1) / nb) * th;
     const int nbands = (oh + band - 1) / band;
-    if ((e = ctx->ensure_events(3 * (size_t)nbands + 1)) != hipSuccess)
+    if ((e = ctx->ensure_events(2 * (size_t)nbands + 1)) != hipSuccess)
         return drain(halide_error_code_device_run_failed, "event creation");
-    hipEvent_t *ev_r = ctx->events.data(), *ev_acc = ev_r + nbands, *ev_out = ev_acc + nbands;
-    hipEvent_t ev_prep = ev_out[nbands];
+    hipEvent_t *ev_r = ctx->events.data(), *ev_acc = ev_r + nbands;
+    hipEvent_t ev_prep = ev_acc[nbands];
     if ((e = hipEventRecord(ev_prep, s1)) != hipSuccess ||
         (e = hipStreamWaitEvent(s3, ev_prep, 0)) != hipSuccess)
         return drain(halide_error_code_device_run_failed, "band ordering");
@@ -603,12 +624,12 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
         if (band_grid(b)) {
             ++g_last_grid_bands;
             rv = plan_accumulate_rows(plan, dr, rmin, rext, row0, nrows,
-                                      (double *)ctx->d_out.p + (size_t)2 * ow * row0, sc, true);
+                                      (double *)ctx->h_out.p + (size_t)2 * ow * row0, sc, true);
             if (rv) {
                 (void)hipStreamSynchronize(s1);
                 (void)hipStreamSynchronize(s2);
                 (void)hipStreamSynchronize(s3);
-                return rv;
+                            return rv;
             }
             if ((e = hipEventRecord(ev_acc[b], sc)) != hipSuccess)
                 return drain(halide_error_code_device_run_failed, "band ordering");
@@ -638,35 +659,25 @@ int backprojection_impl(halide_buffer_t *phs, halide_buffer_t *k_r, int32_t tayl
             (e = hipStreamWaitEvent(sc, ev_r[b], 0)) != hipSuccess)
             return drain(halide_error_code_device_run_failed, "band ordering");
         rv = plan_accumulate_rows(plan, dr, rmin, rext, row0, nrows,
-                                  (double *)ctx->d_out.p + (size_t)2 * ow * row0, sc, false);
+                                  (double *)ctx->h_out.p + (size_t)2 * ow * row0, sc, false);
         if (rv) {
             (void)hipStreamSynchronize(s1);
             (void)hipStreamSynchronize(s2);
             (void)hipStreamSynchronize(s3);
-            return rv;
+                return rv;
         }
         if ((e = hipEventRecord(ev_acc[b], sc)) != hipSuccess)
             return drain(halide_error_code_device_run_failed, "band ordering");
     }
-    for (int b = 0; b < nbands; ++b) {
-        const int row0 = b * band, nrows = std::min(band, oh - row0);
-        const size_t off = sizeof(double) * 2 * (size_t)ow * row0;
-        const size_t n = sizeof(double) * 2 * (size_t)ow * nrows;
-        if ((e = hipStreamWaitEvent(s2, ev_acc[b], 0)) != hipSuccess ||
-            (e = hipMemcpyAsync((uint8_t *)ctx->h_out.p + off, (uint8_t *)ctx->d_out.p + off, n,
-                                hipMemcpyDeviceToHost, s2)) != hipSuccess ||
-            (e = hipEventRecord(ev_out[b], s2)) != hipSuccess)
-            return drain(halide_error_code_copy_to_host_failed, "output download");
-    }
-
-    // Scatter each band {2, ow, nrows} into the caller's output region as its download completes.
+    // Scatter each band {2, ow, nrows} into the caller's output region as its accumulation
+    // completes (the kernel wrote it into h_out).
     const double *res = (const double *)ctx->h_out.p;
     double *dst = (double *)out->host;
     const int c0 = out->dim[0].min, c1 = out->dim[0].min + out->dim[0].extent;
     const bool dense = c0 == 0 && c1 == 2 && out->dim[0].stride == 1 && out->dim[1].stride == 2 &&
                        out->dim[2].stride == 2 * ow;
     for (int b = 0; b < nbands; ++b) {
-        if ((e = hipEventSynchronize(ev_out[b])) != hipSuccess)
+        if ((e = hipEventSynchronize(ev_acc[b])) != hipSuccess)
             return drain(halide_error_code_device_run_failed, "pipeline execution");
         const int row0 = b * band, nrows = std::min(band, oh - row0);
         if (dense) {

@@ -248,6 +248,62 @@ __device__ inline void dft_r<4>(float2 *v) {
     v[3] = make_float2(a1.x - a3.x, a1.y - a3.y);
 }
 
+// forward 16-point DFT in registers as 4 x 4: with n = c + 4 l and k = q + 4 p,
+// X[q + 4 p] = sum_c W_4^(c p) W_16^(c q) sum_l x[c + 4 l] W_4^(l q): four 4-point DFTs over l, the
+// nine non-trivial factors W_16^(c q), four 4-point DFTs over c.
+__device__ inline float2 mul_w16(float2 v, int e) {  // v * W_16^e, e in {1, 2, 3, 6, 9}
+    constexpr float c8 = 0.92387953251128674f, s8 = 0.38268343236508978f, r2 = 0.70710678118654752f;
+    switch (e) {
+        case 1: return cmul(v, make_float2(c8, -s8));
+        case 2: return cmul(v, make_float2(r2, -r2));
+        case 3: return cmul(v, make_float2(s8, -c8));
+        case 6: return cmul(v, make_float2(-r2, -r2));
+        default: return cmul(v, make_float2(-c8, s8));  // e = 9
+    }
+}
+// The 16-point DFT of v, written to out[base + k Ns] (k = 0..15); v is overwritten. The first
+// stage works in place (y[c][q] at v[c + 4 q]), the second writes its outputs straight out.
+__device__ inline void dft16_store(float2 *v, float2 *out, int base, int Ns) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float2 t[4] = {v[c], v[c + 4], v[c + 8], v[c + 12]};
+        dft_r<4>(t);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[c + 4 * q] = t[q];
+    }
+#pragma unroll
+    for (int c = 1; c < 4; ++c)
+#pragma unroll
+        for (int q = 1; q < 4; ++q)
+            v[c + 4 * q] = c * q == 4 ? make_float2(v[c + 4 * q].y, -v[c + 4 * q].x)  // W_16^4 = -i
+                                      : mul_w16(v[c + 4 * q], c * q);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float2 s[4] = {v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
+        dft_r<4>(s);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) out[base + (q + 4 * p) * Ns] = s[p];
+    }
+}
+
+// Pass schedule of the one-row FFTs of N = 2^logN points (fft_pass / fft_pass_oop): a radix-2
+// pass first where logN is odd, then radix 4 while the span Ns is below 32 (at smaller spans a
+// radix-16 pass's strided LDS writes would meet in few banks), then radix 16 while four or more
+// bits are left, then radix 4. `left` = log2(N / Ns). N = 8192: 2, 4, 4, 16, 16 (five LDS passes
+// where radix 4 takes seven).
+__host__ __device__ constexpr int fft_next_radix(int Ns, int left) {
+    return (left & 1) ? 2 : (Ns < 32 || left < 4) ? 4 : 16;
+}
+// Span of the first radix-16 pass (the first span >= 32 of the schedule).
+__host__ __device__ constexpr int fft_first_span16(int logN) { return (logN & 1) ? 32 : 64; }
+
+// Twiddles of a radix-16 pass of span Ns: W_N^(r k N / (16 Ns)) for r = 1..15 and k in [0, Ns), at
+// stage16[(r - 1) Ns + k]; the pass of span Ns at tw + 2N + Ns - fft_first_span16(logN) (after the
+// N-point table and the radix-4 stage tables).
+__device__ inline const float2 *fft_stage16_twiddles(const float2 *tw, int N, int logN, int Ns) {
+    return tw + 2 * N + Ns - fft_first_span16(logN);
+}
+
 // Twiddles of one radix-4 pass of span Ns over an N-point row: W_N^(r k N / (4 Ns)) for r = 1..3
 // and k in [0, Ns), stored at stage[(r - 1) Ns + k] so that the lanes of a wave (consecutive k)
 // read consecutive entries. The N-point table tw (W_N^t, t in [0, N)) is followed by these stage
@@ -300,6 +356,39 @@ __device__ inline void fft_pass(float2 *buf, int N, int Ns, const float2 *__rest
             const int base = (j - k) * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// A radix-16 pass of the same Stockham form: the 16 inputs and 15 twiddles of a butterfly are read
+// before the barrier (lanes past the last butterfly read the last one's); then the 16-point DFT
+// writes its outputs. NB butterflies per
+// thread (N / (16 kFftThreads), rounded up). k = 0 takes no twiddle, as in fft_pass.
+template <int NB>
+__device__ inline void fft_pass16(float2 *buf, int N, int Ns, const float2 *__restrict__ stage,
+                                  int tid) {
+    const int nbf = N / 16;
+    float2 v[NB][16], w[NB][15];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = min(tid + b * kFftThreads, nbf - 1), k = j & (Ns - 1);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[b][r] = buf[j + r * nbf];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) w[b][r - 1] = stage[(r - 1) * Ns + k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = tid + b * kFftThreads;
+        if (j < nbf) {  // whole waves: nbf is a multiple of 64 wherever a radix-16 pass runs
+            const int k = j & (Ns - 1);
+            if (k != 0) {
+#pragma unroll
+                for (int r = 1; r < 16; ++r) v[b][r] = cmul(v[b][r], w[b][r - 1]);
+            }
+            dft16_store(v[b], buf, (j - k) * 16 + k, Ns);
         }
     }
     __syncthreads();
@@ -440,13 +529,18 @@ __device__ inline void range_profile_one(
     QT *__restrict__ q, float *__restrict__ qscale, int tid) {
     load_profile_row<kFftThreads>(buf, phs, k_r, ws, wp, ns, N, profile_source(p, np), tid);
     __syncthreads();
-    int Ns = 1;
-    if (logN & 1) {
-        fft_pass<2, 2 * NB4>(buf, N, 1, tw, tid);
-        Ns = 2;
+    for (int Ns = 1, left = logN; left > 0;) {
+        const int R = fft_next_radix(Ns, left);
+        if (R == 2) {
+            fft_pass<2, 2 * NB4>(buf, N, 1, tw, tid);
+        } else if (R == 4) {
+            fft_pass<4, NB4>(buf, N, Ns, fft_stage_twiddles(tw, N, logN, Ns), tid);
+        } else {
+            fft_pass16<(NB4 + 3) / 4>(buf, N, Ns, fft_stage16_twiddles(tw, N, logN, Ns), tid);
+        }
+        Ns *= R;
+        left -= R == 2 ? 1 : R == 4 ? 2 : 4;
     }
-    for (; Ns < N; Ns *= 4)
-        fft_pass<4, NB4>(buf, N, Ns, fft_stage_twiddles(tw, N, logN, Ns), tid);
     store_profile_row<QT, kFftThreads>(buf, red, N, rot, ext, len, q, qscale, p, tid);
 }
 
@@ -603,6 +697,25 @@ __device__ inline void fft_pass_oop(const float2 *src, float2 *dst, int N, int N
     __syncthreads();
 }
 
+// The radix-16 pass of fft_pass16, out of place, one butterfly at a time (the same arithmetic).
+__device__ inline void fft_pass16_oop(const float2 *src, float2 *dst, int N, int Ns,
+                                      const float2 *__restrict__ stage, int tid) {
+    const int nbf = N / 16;
+#pragma nounroll
+    for (int j = tid; j < nbf; j += kFftSharedThreads) {
+        float2 v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = src[j + r * nbf];
+        const int k = j & (Ns - 1);
+        if (k != 0) {
+#pragma unroll
+            for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], stage[(r - 1) * Ns + k]);
+        }
+        dft16_store(v, dst, (j - k) * 16 + k, Ns);
+    }
+    __syncthreads();
+}
+
 template <typename QT>
 __global__ __launch_bounds__(kFftSharedThreads) void range_profile_shared_kernel(
     const float2 *__restrict__ phs, const float *__restrict__ k_r, const double *__restrict__ ws,
@@ -617,16 +730,16 @@ __global__ __launch_bounds__(kFftSharedThreads) void range_profile_shared_kernel
         load_profile_row<kFftSharedThreads>(a, phs, k_r, ws, wp, ns, N, profile_source(p, np),
                                             tid);
         __syncthreads();
-        int Ns = 1;
-        if (logN & 1) {
-            fft_pass_oop<2>(a, b, N, 1, tw, tid);
-            Ns = 2;
-            float2 *t = a;
-            a = b;
-            b = t;
-        }
-        for (; Ns < N; Ns *= 4) {
-            fft_pass_oop<4>(a, b, N, Ns, fft_stage_twiddles(tw, N, logN, Ns), tid);
+        for (int Ns = 1, left = logN; left > 0;) {
+            const int R = fft_next_radix(Ns, left);
+            if (R == 2)
+                fft_pass_oop<2>(a, b, N, 1, tw, tid);
+            else if (R == 4)
+                fft_pass_oop<4>(a, b, N, Ns, fft_stage_twiddles(tw, N, logN, Ns), tid);
+            else
+                fft_pass16_oop(a, b, N, Ns, fft_stage16_twiddles(tw, N, logN, Ns), tid);
+            Ns *= R;
+            left -= R == 2 ? 1 : R == 4 ? 2 : 4;
             float2 *t = a;
             a = b;
             b = t;
@@ -2528,6 +2641,26 @@ void fill_stage_twiddles(std::vector<float2> &h, size_t base, int n) {
     }
 }
 
+// Stage tables of the radix-16 passes of an n-point FFT (fft_stage16_twiddles), written to
+// h[2n, 3n) from the n-point table in h[0, n): the passes of the schedule fft_next_radix.
+void fill_stage16_twiddles(std::vector<float2> &h, int n) {
+    int logN = 0;
+    while ((1 << logN) < n) ++logN;
+    const int ns16 = fft_first_span16(logN);
+    for (int Ns = 1, left = logN; left > 0;) {
+        const int R = fft_next_radix(Ns, left);
+        if (R == 16) {
+            const size_t stride = (size_t)n / (16 * (size_t)Ns);
+            for (int r = 1; r < 16; ++r)
+                for (int k = 0; k < Ns; ++k)
+                    h[2 * (size_t)n + (size_t)(Ns - ns16) + (size_t)(r - 1) * Ns + k] =
+                        h[(size_t)r * k * stride];
+        }
+        Ns *= R;
+        left -= R == 2 ? 1 : R == 4 ? 2 : 4;
+    }
+}
+
 // In-place forward FFT of a power-of-two length in f64 (iterative radix 2, twiddles evaluated
 // directly), for the host-side Bluestein table.
 void fft_f64_host(std::vector<std::complex<double>> &a) {
@@ -2567,15 +2700,17 @@ const float2 *twiddles(int device, int n_fft, hipStream_t stream) {
     if (it != g_tw.end()) return it->second;
     std::vector<float2> h;
     if ((n_fft & (n_fft - 1)) == 0) {
-        // [0, N): W_N^t; [N, 2N): the stage tables of the radix-4 passes (fft_stage_twiddles),
-        // copies of entries of the first part
+        // [0, N): W_N^t; [N, 2N): the stage tables of the radix-4 passes (fft_stage_twiddles);
+        // [2N, 3N): those of the radix-16 passes (fft_stage16_twiddles); copies of entries of the
+        // first part
         const size_t n = (size_t)n_fft;
-        h.assign(2 * n, make_float2(0.f, 0.f));
+        h.assign(3 * n, make_float2(0.f, 0.f));
         for (int t = 0; t < n_fft; ++t) {
             const double a = -2.0 * M_PI * (double)t / (double)n_fft;
             h[t] = make_float2((float)std::cos(a), (float)std::sin(a));
         }
         fill_stage_twiddles(h, 0, n_fft);
+        fill_stage16_twiddles(h, n_fft);
     } else {
         // Bluestein tables of range_profile_bluestein_kernel, M = 2^ceil(log2(2N - 1)):
         // [0, M) W_M^t, [M, 2M) the stage tables of M, [2M, 3M) FFT_M(b) / M, [3M, 3M + N) the chirp

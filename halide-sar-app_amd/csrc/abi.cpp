@@ -147,7 +147,8 @@ int get_context(Context **out) {
     return 0;
 }
 
-// Host work of a call spread over up to 8 threads: the copies between the caller's buffers and
+// Host work of a call spread over up to 16 threads (at most the usable cores; one per 2 MB of
+// work): the copies between the caller's buffers and
 // the pinned staging (pageable <-> page-locked copies are bound by one core's copy rate otherwise:
 // ~28 GB/s on one thread, ~100 GB/s on eight) and the check of pixel locations against the image
 // plane. The helper threads live for the process (a C2 call makes ~35 such copies; a fresh set of

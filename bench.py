@@ -240,11 +240,12 @@ def main():
         cpu = {"value": round(pp / t_full / 1e9, 5), "unit": "Gpixel·pulses/s",
                "cores": nthreads, "kind": "port",
                "code": ("oracle/sarbp_oracle.c: the reference CPU schedule's loop nest "
-                        "(backprojection.cpp:302-314: rows serial, 4-pixel groups over threads, "
-                        "norm_rr0[4 x np] staged per group) in scalar f64 C with libm sin/cos. "
-                        "The reference vectorises norm_rr0 over pulses and the output over x; "
-                        "its accumulation update (img.inner.update, :306-307) is not vectorised, "
-                        "and runs scalar here as there"),
+                        "(backprojection.cpp:303-314: 4-pixel groups over threads, "
+                        "norm_rr0[np] staged per pixel of the group) in f64 C with libm sin/cos, "
+                        "built -O3 -mavx2: the staged norm_rr0 loop (sqrt included) runs four "
+                        "f64 lanes wide, as the reference vectorises norm_rr0 over pulses "
+                        "(:303-305); its accumulation update (img.inner.update, :309) is not "
+                        "vectorised, and runs scalar here as there"),
                "usable_cores": cpu_info["usable"], "cpu_model": cpu_info["model"],
                "sample": (f"{rows} of {grid['nv']} rows ({rows * grid['nu']} pixels) x "
                           f"{sc.npulses} pulses, median of 3 runs {t_rows:.2f} s "

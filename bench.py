@@ -63,6 +63,8 @@ def main():
                     help="cut the serving stream to its first N jobs (0: the whole stream)")
     ap.add_argument("--serving-load", type=float, default=0.7,
                     help="Poisson arrival rate as a fraction of the measured capacity")
+    ap.add_argument("--published-case", type=int, default=1,
+                    help="also time the sizes of the reference's published GPU run (1) or not (0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this library runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -322,6 +324,10 @@ def main():
         "cpu_baseline": cpu,
         "abi_call": abi,
     }
+    if args.published_case:
+        del out, dB, u8, plan, ds
+        torch.cuda.empty_cache()
+        line["published_case"] = run_published_case(sar, scenes, pipe, torch)
     streams = [c for c in args.serving.split(",") if c and c != "none"]
     if streams:
         line["serving"] = run_serving(args, streams[0])
@@ -346,6 +352,77 @@ def host_cpu() -> dict:
     except OSError:
         pass
     return {"model": model, "usable": usable, "threads": max(1, threads)}
+
+
+# The reference's published GPU figure: `sarbp -s cuda_distributed -t 30 -u 2 -r 0.25` on its
+# Sandia 8192-pulse x 8192-sample data (8192 x 8192 image, N_fft 32768) over 4 Summit nodes with one
+# V100 each; rank 0's backprojection() call returned in 22 014 ms
+# (exp/summit/darpa-demo2/logs/SARBP-Sandia-4.763156:182-197).
+PUBLISHED_GPU = {"value": round(8192.0 ** 3 / 22.014 / 1e9, 2), "unit": "Gpixel·pulses/s",
+                 "hardware": "4 x NVIDIA V100 (Summit, one per node), Halide cuda_distributed",
+                 "call_ms": 22014,
+                 "source": "exp/summit/darpa-demo2/logs/SARBP-Sandia-4.763156:182-197"}
+
+
+def run_published_case(sar, scenes, pipe, torch) -> dict:
+    """The sizes of the reference's published GPU run on one MI355X: 8192 pulses x 8192 samples,
+    an 8192 x 8192 image, Taylor 30, N_fft 32768 (four-step FFT), on a synthetic scene of the same
+    sizes (the reference's Sandia data is not here). Times the device-resident step (HIP events)
+    and backprojection() with host buffers (wall clock, median of 3 after a warm-up call), and
+    compares the two images."""
+    sc = scenes.config_scene("P8192_S8192")
+    grid = pipe.scene_grid(sc, with_pixel_locs=False)
+    ds = pipe.DeviceScene(sc, grid)
+    pp = pipe.pixel_pulses(sc, grid)
+    plan = ds.plan()
+    out = ds.output()
+    stream = torch.cuda.current_stream()
+    for _ in range(2):
+        ds.run(plan, out)
+    torch.cuda.synchronize()
+    res_ms = []
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ds.run(plan, out)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        res_ms.append(e0.elapsed_time(e1))
+    img = out.cpu().numpy()
+    del out, plan
+    phs = ds.phs.cpu().numpy()
+    n_fft = ds.n_fft
+    del ds
+    torch.cuda.empty_cache()
+    plane = sar.image_grid(grid["nu"], grid["nv"], grid["d_u"], grid["d_v"], grid["u_hat"],
+                           grid["v_hat"])
+    out_h = np.empty((grid["nv"], grid["nu"]), np.complex128)
+    call = (phs, sc.k_r, sc.taylor, n_fft, sc.delta_r, plane["u"], plane["v"], sc.pos,
+            plane["pixel_locs"], out_h)
+    rv = sar.backprojection(*call)
+    times = []
+    for _ in range(3):
+        t_a = time.perf_counter()
+        rv = rv or sar.backprojection(*call)
+        times.append((time.perf_counter() - t_a) * 1e3)
+    if rv != 0:
+        raise SystemExit(f"backprojection() failed on {sc.name}: {rv} {sar.last_error()}")
+    call_ms = sorted(times)[1]
+    res_med = sorted(res_ms)[1]
+    agree = float(np.linalg.norm(out_h - img) / np.linalg.norm(img))
+    log(f"bench: {sc.name} resident {res_med:.1f} ms, backprojection() {call_ms:.1f} ms, "
+        f"images agree to {agree:.2e}")
+    return {"workload": (f"{sc.name}: {grid['nu']}x{grid['nv']} pixels from {sc.npulses} pulses x "
+                         f"{sc.nsamples} samples, N_fft {n_fft}, Taylor {sc.taylor}; synthetic "
+                         "scene of the reference's published GPU run's sizes"),
+            "pixel_pulses": pp,
+            "resident_ms": round(res_med, 2), "resident_runs_ms": [round(t, 2) for t in res_ms],
+            "resident_gpixel_pulses_per_s": round(pp / res_med / 1e6, 1),
+            "call_ms": round(call_ms, 2), "call_runs_ms": [round(t, 2) for t in times],
+            "call_gpixel_pulses_per_s": round(pp / call_ms / 1e6, 1),
+            "call_rel_l2_vs_resident": agree,
+            "published": PUBLISHED_GPU,
+            "call_vs_published": round(pp / call_ms / 1e6 / PUBLISHED_GPU["value"], 1)}
 
 
 def run_serving(args, config) -> dict:

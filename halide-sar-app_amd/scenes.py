@@ -113,13 +113,18 @@ def linear_spotlight_scene(npulses=2000, nsamples=2048, f_c=10e9, bw=1e9, stando
 
 
 def config_scene(name: str) -> Scene:
-    """The named benchmark configurations C1, C2 and C3."""
+    """The named benchmark configurations C1, C2 and C3, and P8192_S8192: the sizes of the
+    reference's published distributed-CUDA run (exp/summit/darpa-demo2/logs/SARBP-Sandia-4.763156:
+    182-197: 8192 pulses x 8192 samples, an 8192 x 8192 image, Taylor 30, upsample 2, so N_fft
+    32768) on a synthetic scene of the C2/C3 family."""
     if name == "C1":
         return circular_arc_scene()
     if name == "C2":
         return linear_spotlight_scene()
-    if name == "C3":
-        sc = linear_spotlight_scene(npulses=8000, nsamples=4096, taylor=30, seed=1003, name="C3")
+    if name in ("C3", "P8192_S8192"):
+        c3 = name == "C3"
+        sc = linear_spotlight_scene(npulses=8000 if c3 else 8192, nsamples=4096 if c3 else 8192,
+                                    taylor=30, seed=1003 if c3 else 1006, name=name)
         du = sc.delta_r * sc.nsamples / 8192
         sc.image = (8192, 8192, du, du)
         return sc

@@ -646,6 +646,45 @@ def test_c3_full_size_sampled(sar, orc, scenes, pipe, torch_dev):
         assert e < tol, name
 
 
+def test_published_run_sizes_sampled(sar, orc, scenes, pipe, torch_dev):
+    """The sizes of the reference's published GPU run (SARBP-Sandia-4.763156:182-197: 8192 pulses x
+    8192 samples, an 8192^2 image, Taylor 30, upsample 2, so N_fft 32768 and the four-step FFT) on
+    the synthetic P8192_S8192 scene, pixel locations generated on the device. Oracle on 1032
+    sampled pixels: the four corners' neighbourhoods, every 16th pixel of one row, random ones."""
+    torch = torch_dev
+    sc = scenes.config_scene("P8192_S8192")
+    grid = pipe.scene_grid(sc, with_pixel_locs=False)
+    ds = pipe.DeviceScene(sc, grid)
+    assert ds.n_fft == 32768
+    nu, nv = grid["nu"], grid["nv"]
+    rng = np.random.default_rng(13)
+    xs = np.concatenate([[0, 1, nu - 1, nu - 2, 0, nu - 1, 0, nu - 1], np.arange(0, nu, 16),
+                         rng.integers(0, nu, 512)])
+    ys = np.concatenate([[0, 0, 0, 0, nv - 1, nv - 1, 1, nv - 2], np.full(nu // 16, 2 * nv // 3),
+                         rng.integers(0, nv, 512)])
+    plan = ds.plan()
+    out = ds.output()
+    print("P8192_S8192: running 8192^2 x 8192, N_fft 32768", flush=True)
+    ds.run(plan, out)
+    torch.cuda.synchronize()
+    yi, xi = torch.from_numpy(ys).to(ds.dev), torch.from_numpy(xs).to(ds.dev)
+    got = out[yi, xi].cpu().numpy()
+    assert np.all(np.isfinite(got))
+    phs = ds.phs.cpu().numpy()
+    del out, plan, ds
+    torch.cuda.empty_cache()
+    Q = orc.range_profiles(phs, sc.k_r, sc.taylor, 32768)
+    _, nr0 = orc.norm_r0(sc.pos)
+    pts = sar.grid_points(grid, xs, ys)
+    cnt = len(xs)
+    ref = orc.backproject_points(Q, sc.nsamples, sc.delta_r, sc.k_r[sc.nsamples // 2], sc.pos,
+                                 nr0, pts, cnt, 1, np.arange(cnt, dtype=np.int32),
+                                 np.zeros(cnt, np.int32), nthreads=NTHREADS)
+    e = rel_l2(got, ref)
+    print(f"P8192_S8192: sampled rel_l2 {e:.3e} over {cnt} pixels")
+    assert e < TOL
+
+
 def test_img_output_parity(sar, orc, torch_dev):
     torch = torch_dev
     g = np.load(os.path.join(GOLDEN, "c1_64.npz"))

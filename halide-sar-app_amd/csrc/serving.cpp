@@ -528,12 +528,35 @@ int band_rows(const BpArgs &a) {
     return (int)(std::max<int64_t>(1, kBandPixels / (16 * th * cols)) * th);
 }
 
-// Enqueue bands while fewer than kBandsAhead are outstanding; after the last band, ev_acc.
+// Pixel-pulses of a banded job's rows not yet enqueued.
+int64_t remaining_pp(const Job &j) {
+    return (int64_t)(j.args.height - j.next_row) * j.args.width * j.d.npulses;
+}
+
+// Whether j may enqueue bands now: of the banded jobs on its stream, the one with the least work
+// left (the earlier job on a tie). The others wait with at most the bands they already have in
+// the queue, so jobs sharing a stream finish shortest-remaining-first rather than all late
+// together; a job becomes the leader when the one before it has enqueued its last band.
+bool stream_leader(const sarbp_server *s, const Job &j) {
+    const int64_t mine = remaining_pp(j);
+    for (const auto &op : s->inflight) {
+        const Job &o = *op;
+        if (&o == &j || o.stream != j.stream || o.phase != Phase::Computing || !o.banding)
+            continue;
+        const int64_t theirs = remaining_pp(o);
+        if (theirs < mine || (theirs == mine && o.id < j.id)) return false;
+    }
+    return true;
+}
+
+// Enqueue bands while fewer than kBandsAhead are outstanding (and j leads its stream); after the
+// last band, ev_acc.
 int advance_bands(sarbp_server *s, Job &j) {
     hipStream_t st = s->streams[j.stream];
     while (j.bands_done < j.bands_enqueued &&
            hipEventQuery(j.ev_band[j.bands_done % kBandsAhead]) == hipSuccess)
         ++j.bands_done;
+    if (s->cfg.band_order == 0 && !stream_leader(s, j)) return 0;
     const int rows = band_rows(j.args);
     hipError_t e = hipSuccess;
     while (j.next_row < j.args.height && j.bands_enqueued - j.bands_done < kBandsAhead) {
@@ -909,9 +932,9 @@ extern "C" int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server 
     }
     *out = nullptr;
     if (cfg->max_inflight < 1 || cfg->num_streams < 1 || cfg->num_streams > 16 ||
-        cfg->hbm_budget_bytes <= 0) {
-        set_error("sarbp_server_create: max_inflight >= 1, 1 <= num_streams <= 16 and a positive "
-                  "budget are required");
+        cfg->hbm_budget_bytes <= 0 || cfg->band_order < 0 || cfg->band_order > 1) {
+        set_error("sarbp_server_create: max_inflight >= 1, 1 <= num_streams <= 16, a positive "
+                  "budget and band_order 0 or 1 are required");
         return halide_error_code_constraint_violated;
     }
     if (sarbp_device_count() <= cfg->device || cfg->device < 0) {

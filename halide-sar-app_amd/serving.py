@@ -49,7 +49,7 @@ _P, _i32, _i64, _f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", _i32), ("max_inflight", _i32), ("hbm_budget_bytes", _i64),
                 ("fuse_max_pixel_pulses", _i64), ("num_streams", _i32), ("autotune", _i32),
-                ("fuse_launch_pixel_pulses", _i64), ("reserved", _i32 * 6)]
+                ("fuse_launch_pixel_pulses", _i64), ("band_order", _i32), ("reserved", _i32 * 5)]
 
 
 class JobDesc(ctypes.Structure):
@@ -85,7 +85,7 @@ class ServingError(RuntimeError):
 
 class Server:
     def __init__(self, device=0, max_inflight=64, hbm_budget_gb=64.0, fuse_max_pixel_pulses=2 ** 31,
-                 num_streams=2, autotune=True, fuse_launch_pixel_pulses=None):
+                 num_streams=2, autotune=True, fuse_launch_pixel_pulses=None, band_order=0):
         c = ServerConfig()
         _lib.sarbp_server_config_init(ctypes.byref(c))
         c.device, c.max_inflight = device, max_inflight
@@ -94,6 +94,7 @@ class Server:
         c.num_streams, c.autotune = num_streams, int(bool(autotune))
         if fuse_launch_pixel_pulses is not None:
             c.fuse_launch_pixel_pulses = int(fuse_launch_pixel_pulses)
+        c.band_order = int(band_order)
         self.cfg = c
         self._s = _P()
         rv = _lib.sarbp_server_create(ctypes.byref(c), ctypes.byref(self._s))
@@ -449,6 +450,8 @@ def main(argv=None):
     ap.add_argument("--streams", type=int, default=2, help="large-job streams of the engine")
     ap.add_argument("--fuse-max", type=float, default=2 ** 31,
                     help="largest job (pixel-pulses) the engine fuses")
+    ap.add_argument("--band-order", type=int, default=0, choices=[0, 1],
+                    help="larger jobs sharing a stream: 0 least work left first, 1 interleaved")
     ap.add_argument("--hw-queues", type=int, default=ENGINE_HW_QUEUES,
                     help="GPU_MAX_HW_QUEUES for this process (HIP's default is 4)")
     a = ap.parse_args(argv)
@@ -456,7 +459,8 @@ def main(argv=None):
     if a.jobs:
         specs = specs[:a.jobs]
     res = benchmark(specs, load=a.load, log=lambda m: print(m, file=sys.stderr, flush=True),
-                    num_streams=a.streams, fuse_max_pixel_pulses=a.fuse_max)
+                    num_streams=a.streams, fuse_max_pixel_pulses=a.fuse_max,
+                    band_order=a.band_order)
     print(json.dumps(dict(config=a.config, jobs=len(specs), **res)), flush=True)
 
 

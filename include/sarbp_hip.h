@@ -282,7 +282,10 @@ typedef struct sarbp_server_config {
     int64_t fuse_launch_pixel_pulses; /* pixel-pulses of one fused launch at most, which bounds
                                          how long other jobs' kernels wait behind one (0: the
                                          default, 8e9, ~4 ms on an idle device) */
-    int32_t reserved[6];
+    int32_t band_order;             /* larger jobs sharing a stream: 0 (default) the one with the
+                                       least work left enqueues its bands, the others wait; 1 all
+                                       of them enqueue, their bands interleaving */
+    int32_t reserved[5];
 } sarbp_server_config;
 
 typedef struct sarbp_job {

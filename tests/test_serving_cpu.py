@@ -84,6 +84,9 @@ def test_submit_rejects_null_server_and_job(sar):
     lib.sarbp_server_config_init(ctypes.byref(cfg))
     assert (cfg.max_inflight, cfg.num_streams, cfg.autotune) == (64, 2, 1)
     assert cfg.hbm_budget_bytes == 64 << 30
+    assert cfg.band_order == 0  # least work left first
     cfg.num_streams = 0
     h = ctypes.c_void_p()
+    assert lib.sarbp_server_create(ctypes.byref(cfg), ctypes.byref(h)) == sar.ERR["constraint_violated"]
+    cfg.num_streams, cfg.band_order = 2, 2
     assert lib.sarbp_server_create(ctypes.byref(cfg), ctypes.byref(h)) == sar.ERR["constraint_violated"]

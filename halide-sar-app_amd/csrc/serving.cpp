@@ -63,6 +63,9 @@ constexpr int kFuseSlots = 2;                       // fused launches in flight 
 // the tile the tuner picks.
 constexpr int64_t kBandPixels = 4096 * 16 * 32;
 constexpr int kBandsAhead = 2;
+// A banded job that has enqueued no band for this long goes ahead of the least-work-left order
+// (oldest first), so that no job waits without bound behind a run of smaller ones.
+constexpr int64_t kBandPatienceNs = 50'000'000;  // 50 ms
 // Four chunks in flight (~0.6 ms of copying) keep the copy engine busy across a scheduler pass
 // over many jobs; with two, a large job's output took 2-5x as long when 2000 jobs were in flight.
 constexpr size_t kUploadChunkBytes = (size_t)8 << 20;  // ~0.15 ms over PCIe 5 x16
@@ -210,6 +213,7 @@ struct Job {
     hipEvent_t ev_band[2] = {};     // large: completion of the outstanding accumulation bands
     bool banding = false;           // large: bands still to enqueue
     int next_row = 0, bands_enqueued = 0, bands_done = 0;
+    int64_t t_band = 0;  // when the job last enqueued a band (or began banding)
     hipEvent_t ev_done = nullptr;   // output in host memory
     hipEvent_t ev_begin = nullptr;  // timing pair with ev_done: device time of the job
     int64_t host_ns = 0;            // scheduler time spent enqueuing the job's work
@@ -536,15 +540,19 @@ int64_t remaining_pp(const Job &j) {
 // Whether j may enqueue bands now: of the banded jobs on its stream, the one with the least work
 // left (the earlier job on a tie). The others wait with at most the bands they already have in
 // the queue, so jobs sharing a stream finish shortest-remaining-first rather than all late
-// together; a job becomes the leader when the one before it has enqueued its last band.
-bool stream_leader(const sarbp_server *s, const Job &j) {
-    const int64_t mine = remaining_pp(j);
+// together; a job becomes the leader when the one before it has enqueued its last band. Jobs that
+// have enqueued nothing for kBandPatienceNs come first, the longest waiting first.
+bool stream_leader(const sarbp_server *s, const Job &j, int64_t now) {
+    auto key = [now](const Job &x) {
+        const bool waited = now - x.t_band > kBandPatienceNs;
+        return std::make_tuple(waited ? 0 : 1, waited ? x.t_band : remaining_pp(x), x.id);
+    };
+    const auto mine = key(j);
     for (const auto &op : s->inflight) {
         const Job &o = *op;
         if (&o == &j || o.stream != j.stream || o.phase != Phase::Computing || !o.banding)
             continue;
-        const int64_t theirs = remaining_pp(o);
-        if (theirs < mine || (theirs == mine && o.id < j.id)) return false;
+        if (key(o) < mine) return false;
     }
     return true;
 }
@@ -556,7 +564,7 @@ int advance_bands(sarbp_server *s, Job &j) {
     while (j.bands_done < j.bands_enqueued &&
            hipEventQuery(j.ev_band[j.bands_done % kBandsAhead]) == hipSuccess)
         ++j.bands_done;
-    if (s->cfg.band_order == 0 && !stream_leader(s, j)) return 0;
+    if (s->cfg.band_order == 0 && !stream_leader(s, j, now_ns(s->t0))) return 0;
     const int rows = band_rows(j.args);
     hipError_t e = hipSuccess;
     while (j.next_row < j.args.height && j.bands_enqueued - j.bands_done < kBandsAhead) {
@@ -573,6 +581,7 @@ int advance_bands(sarbp_server *s, Job &j) {
         }
         j.next_row += a.height;
         ++j.bands_enqueued;
+        j.t_band = now_ns(s->t0);
     }
     if (j.next_row >= j.args.height && j.banding) {
         j.banding = false;
@@ -634,6 +643,7 @@ int run_alone(sarbp_server *s, Job &j) {
     j.t_launch = now_ns(s->t0);
     if (!j.t_ready) j.t_ready = j.t_launch;
     j.banding = true;
+    j.t_band = j.t_launch;
     j.next_row = 0;
     for (int k = 0; k < kBandsAhead; ++k)
         if (!j.ev_band[k] && hipEventCreateWithFlags(&j.ev_band[k], hipEventDisableTiming) != hipSuccess) {

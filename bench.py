@@ -198,7 +198,7 @@ def main():
         # five warm-up calls, then the median of five. The first call creates the plan and the
         # staging and touches the host pages; the GPU clock then takes a few calls to reach its
         # level under this duty cycle (each call leaves the GPU idle while phs is uploaded): the
-        # calls run ~6.7, 6.5, 6.4, 6.2, 5.9 ms, then ~5.8 ms on one box
+        # calls ran ~6.7, 6.5, 6.4, 6.2, 5.9 ms, then ~5.8 ms, on one MI355X
         warm, times = [], []
         rv = 0
         for i in range(10):

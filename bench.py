@@ -434,7 +434,7 @@ def run_serving(args, config) -> dict:
     a process's streams onto its GPU_MAX_HW_QUEUES hardware queues in turn, and the streams that
     this process's other legs created earlier shifted the engine's streams onto shared queues: the
     fused launches then queued behind a large job's bands (C5 at 70 %: p50 / p99 of 16.1 / 26.7 ms
-    here against 1.9 / 5.7 ms in a process of its own, one box)."""
+    here against 1.9 / 5.7 ms in a process of its own, on the same MI355X)."""
     if config not in ("C4", "C5"):
         raise SystemExit(f"bench.py: unknown serving stream {config!r} (C4, C5 or none)")
     cmd = [sys.executable, "-u", os.path.join(ROOT, "halide-sar-app_amd", "serving.py"),
